@@ -1,0 +1,479 @@
+// C ABI of the gfx950 elas3dxtal_pcg solver (include/e3x.h).
+#include "e3x.h"
+#include "e3x_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace e3x;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define E3X_HIP(call)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(e_ == hipErrorOutOfMemory ? E3X_ENOMEM : E3X_EDEVICE,                   \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(Solver &s, T **p, int64_t n)
+{
+    *p = nullptr;
+    if (n <= 0) return hipSuccess;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+    if (e == hipSuccess) s.bytes += (int64_t)sizeof(T) * n;
+    return e;
+}
+
+void release(Solver &s)
+{
+    (void)hipSetDevice(s.device);
+    void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.u, s.g, s.b, s.h, s.z,
+                   s.ah, s.minv, s.g32, s.h32, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
+                   s.counter, s.st, s.hist, s.results, s.counts, s.post};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (s.results_host) (void)hipHostFree(s.results_host);
+    if (s.flags_host) (void)hipHostFree(s.flags_host);
+    if (s.st_host) (void)hipHostFree(s.st_host);
+    for (hipEvent_t &e : s.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+}
+
+int check_handle(const e3x_solver *h)
+{
+    if (!h) return fail(E3X_EINVAL, "NULL solver handle");
+    return E3X_OK;
+}
+
+Solver &S(e3x_solver *h) { return *reinterpret_cast<Solver *>(h); }
+
+int ensure_mixed_buffers(Solver &s)
+{
+    if (s.g32) return E3X_OK;
+    const int64_t nv = (int64_t)s.nrhs * 3 * s.ns;
+    E3X_HIP(dalloc(s, &s.g32, nv));
+    E3X_HIP(dalloc(s, &s.h32, nv));
+    E3X_HIP(dalloc(s, &s.z32, nv));
+    E3X_HIP(dalloc(s, &s.ah32, nv));
+    E3X_HIP(dalloc(s, &s.minv32, 6 * s.ns));
+    return E3X_OK;
+}
+
+int ensure_hist(Solver &s, int ldemb)
+{
+    const int64_t need = (int64_t)s.nrhs * std::max(ldemb, 1);
+    if (need <= s.hist_cap) return E3X_OK;
+    if (s.hist) {
+        (void)hipFree(s.hist);
+        s.bytes -= (int64_t)sizeof(double) * s.hist_cap;
+        s.hist = nullptr;
+    }
+    E3X_HIP(dalloc(s, &s.hist, need));
+    s.hist_cap = need;
+    return E3X_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *e3x_last_error(void) { return g_err.c_str(); }
+
+int e3x_version(void) { return 10000; }
+
+int e3x_device_count(int *count)
+{
+    if (!count) return fail(E3X_EINVAL, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+    if (e != hipSuccess && e != hipErrorNoDevice) return fail(E3X_EDEVICE, hipGetErrorString(e));
+    return E3X_OK;
+}
+
+int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_solver **out)
+{
+    if (!out) return fail(E3X_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (nx < 2 || ny < 2 || nz < 2) return fail(E3X_EINVAL, "grid extents must be >= 2");
+    if ((int64_t)nx * ny * nz > (int64_t)INT32_MAX) return fail(E3X_EINVAL, "grid exceeds 2^31 nodes");
+    if (nphase < 1 || nphase > 65535) return fail(E3X_EINVAL, "nphase must be in [1, 65535]");
+    if (nrhs < 1 || nrhs > kMaxRhs) return fail(E3X_EINVAL, "nrhs must be in [1, 6]");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(E3X_EDEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(E3X_EINVAL, "device index out of range");
+    Solver *s = new (std::nothrow) Solver();
+    if (!s) return fail(E3X_ENOMEM, "host allocation failed");
+    s->nx = nx; s->ny = ny; s->nz = nz; s->nphase = nphase; s->nrhs = nrhs; s->device = device;
+    s->ns = (int64_t)nx * ny * nz;
+    auto bail = [&](int code) {
+        release(*s);
+        delete s;
+        return code;
+    };
+#define CK(call)                                                                                \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return bail(fail(e_ == hipErrorOutOfMemory ? E3X_ENOMEM : E3X_EDEVICE,              \
+                             std::string(#call) + ": " + hipGetErrorString(e_)));               \
+    } while (0)
+    CK(hipSetDevice(device));
+    CK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (hipEvent_t &e : s->ev) CK(hipEventCreate(&e));
+    const int64_t ns = s->ns, nv = (int64_t)nrhs * 3 * ns;
+    CK(dalloc(*s, &s->cmod, (int64_t)nphase * 36));
+    CK(dalloc(*s, &s->dk, (int64_t)nphase * 576));
+    CK(dalloc(*s, &s->sten, (int64_t)nphase * 243));
+    CK(dalloc(*s, &s->dk32, (int64_t)nphase * 576));
+    CK(dalloc(*s, &s->sten32, (int64_t)nphase * 243));
+    CK(dalloc(*s, &s->orient, (int64_t)3 * std::max(nphase - 1, 1)));
+    CK(dalloc(*s, &s->pix, ns));
+    CK(dalloc(*s, &s->u, nv));
+    CK(dalloc(*s, &s->g, nv));
+    CK(dalloc(*s, &s->b, nv));
+    CK(dalloc(*s, &s->h, nv));
+    CK(dalloc(*s, &s->z, nv));
+    CK(dalloc(*s, &s->ah, nv));
+    CK(dalloc(*s, &s->minv, 6 * ns));
+    CK(dalloc(*s, &s->cconst, kMaxRhs));
+    s->partials_cap = std::max<int64_t>(12 * (int64_t)nrhs * vector_blocks(*s), 2 * (int64_t)nrhs * stencil_blocks(*s));
+    s->partials_cap = std::max<int64_t>(s->partials_cap, (int64_t)kMaxRhs * vector_blocks(*s));
+    CK(dalloc(*s, &s->partials, s->partials_cap));
+    CK(dalloc(*s, &s->counter, 1));
+    CK(hipMemset(s->counter, 0, sizeof(unsigned)));
+    CK(dalloc(*s, &s->st, kMaxRhs));
+    CK(dalloc(*s, &s->results, 12 * kMaxRhs));
+    CK(dalloc(*s, &s->counts, nphase));
+    CK(hipHostMalloc((void **)&s->results_host, sizeof(double) * 12 * kMaxRhs, hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&s->flags_host, sizeof(int) * 2 * kMaxRhs, hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&s->st_host, sizeof(PcgState) * 3 * kMaxRhs, hipHostMallocDefault));
+    CK(hipMemset(s->h, 0, sizeof(double) * (size_t)nv));
+    CK(hipMemset(s->u, 0, sizeof(double) * (size_t)nv));
+#undef CK
+    *out = reinterpret_cast<e3x_solver *>(s);
+    return E3X_OK;
+}
+
+int e3x_destroy(e3x_solver *h)
+{
+    if (!h) return E3X_OK;
+    Solver *s = reinterpret_cast<Solver *>(h);
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    release(*s);
+    delete s;
+    return E3X_OK;
+}
+
+int e3x_memory_bytes(const e3x_solver *h, int64_t *bytes)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (!bytes) return fail(E3X_EINVAL, "bytes is NULL");
+    *bytes = reinterpret_cast<const Solver *>(h)->bytes;
+    return E3X_OK;
+}
+
+int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orientation)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!pix) return fail(E3X_EINVAL, "pix is NULL");
+    std::vector<uint16_t> p16((size_t)s.ns);
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t m = 0; m < s.ns; m++) {
+        lo = std::min(lo, pix[m]);
+        hi = std::max(hi, pix[m]);
+    }
+    // f90:2986-2989: labels must lie in [1, nphase]
+    if (lo < 1 || hi > s.nphase)
+        return fail(E3X_EPHASE, "phase label in pix out of bounds: min " + std::to_string(lo) + ", max " +
+                                    std::to_string(hi) + ", nphase " + std::to_string(s.nphase));
+    for (int64_t m = 0; m < s.ns; m++) p16[(size_t)m] = (uint16_t)(pix[m] - 1);
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(hipMemcpyAsync(s.pix, p16.data(), sizeof(uint16_t) * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
+    if (orientation && s.nphase > 1)
+        E3X_HIP(hipMemcpyAsync(s.orient, orientation, sizeof(double) * 3 * (size_t)(s.nphase - 1),
+                               hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    s.have_pix = true;
+    s.have_femat = s.have_precond = s.have_precond32 = false;
+    return E3X_OK;
+}
+
+int e3x_assig(e3x_solver *h, double *prob)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!prob) return fail(E3X_EINVAL, "prob is NULL");
+    if (!s.have_pix) return fail(E3X_ESTATE, "e3x_assig before e3x_set_microstructure");
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(launch_assig(s));
+    std::vector<unsigned> cnt((size_t)s.nphase);
+    E3X_HIP(hipMemcpyAsync(cnt.data(), s.counts, sizeof(unsigned) * s.nphase, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    for (int p = 0; p < s.nphase; p++) prob[p] = (double)cnt[(size_t)p] / (double)s.ns;
+    return E3X_OK;
+}
+
+int e3x_femat(e3x_solver *h, double c11, double c12, double c44, const double *phasemod, int flag_m,
+              const double *strain6, double *C_out)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_pix) return fail(E3X_ESTATE, "e3x_femat before e3x_set_microstructure");
+    if (flag_m != 0 && flag_m != 1) return fail(E3X_EINVAL, "flag_m must be 0 or 1");
+    if (flag_m == 0 && s.nphase != 2) return fail(E3X_EINVAL, "flag_m = 0 needs nphase = 2");
+    if (!strain6) return fail(E3X_EINVAL, "strain6 is NULL");
+    double pm[4] = {0, 0, 0, 0};
+    if (phasemod) {
+        pm[0] = phasemod[0];
+        pm[1] = phasemod[1];
+        if (flag_m == 0) {
+            pm[2] = phasemod[2];
+            pm[3] = phasemod[3];
+        }
+    }
+    s.flag_m = flag_m;
+    for (int q = 0; q < 6 * kMaxRhs; q++) s.strain[q] = q < 6 * s.nrhs ? strain6[q] : 0.0;
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(launch_femat(s, c11, c12, c44, pm));
+    E3X_HIP(launch_rhs(s));
+    if (C_out) {
+        E3X_HIP(hipMemcpyAsync(s.results_host, s.cconst, sizeof(double) * s.nrhs, hipMemcpyDeviceToHost, s.stream));
+        E3X_HIP(hipStreamSynchronize(s.stream));
+        for (int r = 0; r < s.nrhs; r++) C_out[r] = s.results_host[r];
+    } else {
+        E3X_HIP(hipStreamSynchronize(s.stream));
+    }
+    s.have_femat = true;
+    s.have_precond = s.have_precond32 = false;
+    return E3X_OK;
+}
+
+int e3x_affine_guess(e3x_solver *h)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_femat) return fail(E3X_ESTATE, "e3x_affine_guess before e3x_femat");
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(launch_affine(s));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    return E3X_OK;
+}
+
+int e3x_energy(e3x_solver *h, double *utot, double *gg)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_femat) return fail(E3X_ESTATE, "e3x_energy before e3x_femat");
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(launch_energy(s));
+    E3X_HIP(hipMemcpyAsync(s.results_host, s.results, sizeof(double) * 2 * s.nrhs, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    for (int r = 0; r < s.nrhs; r++) {
+        if (utot) utot[r] = s.results_host[r];
+        if (gg) gg[r] = s.results_host[s.nrhs + r];
+    }
+    return E3X_OK;
+}
+
+int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *gg0, float tol, int ldemb,
+                 int kkk, int *lstep, double *gg, double *history, int *exit_flag, int *reset_count)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_femat) return fail(E3X_ESTATE, "e3x_dembx before e3x_femat");
+    if (!gg0) return fail(E3X_EINVAL, "gg0 is NULL");
+    if (precision != E3X_PREC_FP64 && precision != E3X_PREC_MIXED)
+        return fail(E3X_EINVAL, "precision must be E3X_PREC_FP64 or E3X_PREC_MIXED");
+    if (ldemb < 0) return fail(E3X_EINVAL, "ldemb must be >= 0");
+    if (replace_every < 0) return fail(E3X_EINVAL, "replace_every must be >= 0");
+    const bool mixed = precision == E3X_PREC_MIXED;
+    E3X_HIP(hipSetDevice(s.device));
+    if (mixed)
+        if (int rc = ensure_mixed_buffers(s)) return rc;
+    if (int rc = ensure_hist(s, ldemb)) return rc;
+    const int64_t nv = (int64_t)s.nrhs * 3 * s.ns;
+
+    E3X_HIP(hipEventRecord(s.ev[0], s.stream));
+    // block-Jacobi inverse, built once per femat (f90:1498-1520)
+    const bool need32 = mixed && !s.have_precond32;
+    if (!s.have_precond || kkk == 1 || need32) {
+        E3X_HIP(launch_precond(s, mixed || s.have_precond32));
+        s.have_precond = true;
+        if (mixed) s.have_precond32 = true;
+    }
+    E3X_HIP(hipEventRecord(s.ev[1], s.stream));
+    if (mixed) {
+        E3X_HIP(launch_to_f32(s.g, s.g32, nv, s.stream));
+        if (kkk != 1) E3X_HIP(launch_to_f32(s.h, s.h32, nv, s.stream));
+    }
+    E3X_HIP(launch_pcg_init(s, mixed, kkk, ldemb, tol, gg0));
+
+    // PCG loop: every kernel reads the per-rhs stop state on the device, so the host
+    // only polls (double-buffered, one chunk behind) to stop launching.
+    const int chunk = s.ns >= (1 << 21) ? 4 : (s.ns >= (1 << 17) ? 16 : 32);
+    int it = 1, k = 0;
+    int *flags[2] = {s.flags_host, s.flags_host + kMaxRhs};
+    while (it <= ldemb) {
+        const int n = std::min(chunk, ldemb - it + 1);
+        for (int t = 0; t < n; t++) {
+            const int itt = it + t;
+            const bool replace = mixed && replace_every > 0 && itt % replace_every == 0;
+            E3X_HIP(launch_pcg_iter(s, mixed, itt, replace));
+        }
+        it += n;
+        // copy the stop_iter of every rhs into the pinned slot of this chunk
+        for (int r = 0; r < s.nrhs; r++)
+            E3X_HIP(hipMemcpyAsync(flags[k & 1] + r, &s.st[r].stop_iter, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+        E3X_HIP(hipEventRecord(s.ev[2 + (k & 1)], s.stream));
+        if (k > 0) {
+            E3X_HIP(hipEventSynchronize(s.ev[2 + ((k - 1) & 1)]));
+            bool all = true;
+            for (int r = 0; r < s.nrhs; r++) all &= flags[(k - 1) & 1][r] != 0;
+            if (all) break;
+        }
+        k++;
+    }
+    E3X_HIP(hipEventRecord(s.ev[2], s.stream));
+    if (mixed) {
+        E3X_HIP(launch_to_f64(s.g32, s.g, nv, s.stream));
+        E3X_HIP(launch_to_f64(s.h32, s.h, nv, s.stream));
+    }
+    E3X_HIP(hipMemcpyAsync(s.st_host, s.st, sizeof(PcgState) * s.nrhs, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    float ms0 = 0, ms1 = 0;
+    (void)hipEventElapsedTime(&ms0, s.ev[0], s.ev[1]);
+    (void)hipEventElapsedTime(&ms1, s.ev[1], s.ev[2]);
+    s.last_precond_ms = ms0;
+    s.last_pcg_ms = ms1;
+    for (int r = 0; r < s.nrhs; r++) {
+        const PcgState &st = s.st_host[r];
+        const int L = st.stop_iter > 0 ? st.stop_iter : 0;
+        if (lstep) lstep[r] = L;
+        if (gg) gg[r] = st.gg;
+        if (exit_flag) exit_flag[r] = st.exit_flag;
+        if (reset_count) reset_count[r] = st.reset_count;
+        if (history && L > 0) {
+            E3X_HIP(hipMemcpy(history + (int64_t)r * ldemb, s.hist + (int64_t)r * ldemb, sizeof(double) * L,
+                              hipMemcpyDeviceToHost));
+            for (int q = L; q < ldemb; q++) history[(int64_t)r * ldemb + q] = NAN;
+        } else if (history) {
+            for (int q = 0; q < ldemb; q++) history[(int64_t)r * ldemb + q] = NAN;
+        }
+    }
+    return E3X_OK;
+}
+
+int e3x_dembx(e3x_solver *h, const double *gg0, float tol, int ldemb, int kkk, int *lstep, double *gg,
+              double *history, int *exit_flag, int *reset_count)
+{
+    return e3x_dembx_ex(h, E3X_PREC_FP64, 0, gg0, tol, ldemb, kkk, lstep, gg, history, exit_flag, reset_count);
+}
+
+int e3x_stress(e3x_solver *h, double *stress6, double *strain6)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_femat) return fail(E3X_ESTATE, "e3x_stress before e3x_femat");
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(launch_stress(s));
+    E3X_HIP(hipMemcpyAsync(s.results_host, s.results, sizeof(double) * 12 * s.nrhs, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    for (int r = 0; r < s.nrhs; r++)
+        for (int q = 0; q < 6; q++) {
+            if (stress6) stress6[6 * r + q] = s.results_host[12 * r + q];
+            if (strain6) strain6[6 * r + q] = s.results_host[12 * r + 6 + q];
+        }
+    return E3X_OK;
+}
+
+int e3x_stress_fullfield(e3x_solver *h, int rhs, double *stress, double *vm)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_femat) return fail(E3X_ESTATE, "e3x_stress_fullfield before e3x_femat");
+    if (rhs < 0 || rhs >= s.nrhs) return fail(E3X_EINVAL, "rhs out of range");
+    E3X_HIP(hipSetDevice(s.device));
+    if (!s.post) E3X_HIP(dalloc(s, &s.post, 7 * s.ns));  // staging: stress (6 ns) then vm (ns)
+    double *dsf = stress ? s.post : nullptr, *dvm = vm ? s.post + 6 * s.ns : nullptr;
+    E3X_HIP(launch_fullfield(s, rhs, dsf, dvm));
+    if (stress)
+        E3X_HIP(hipMemcpyAsync(stress, dsf, sizeof(double) * 6 * (size_t)s.ns, hipMemcpyDeviceToHost, s.stream));
+    if (vm) E3X_HIP(hipMemcpyAsync(vm, dvm, sizeof(double) * (size_t)s.ns, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    return E3X_OK;
+}
+
+int e3x_get_field(e3x_solver *h, int which, int rhs, double *host)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!host) return fail(E3X_EINVAL, "host is NULL");
+    if (rhs < 0 || rhs >= s.nrhs) return fail(E3X_EINVAL, "rhs out of range");
+    const double *src = which == E3X_FIELD_U ? s.u : which == E3X_FIELD_GB ? s.g : which == E3X_FIELD_B ? s.b
+                        : which == E3X_FIELD_H ? s.h : nullptr;
+    if (!src) return fail(E3X_EINVAL, "unknown field");
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(hipMemcpyAsync(host, src + (int64_t)rhs * 3 * s.ns, sizeof(double) * 3 * (size_t)s.ns,
+                           hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    return E3X_OK;
+}
+
+int e3x_set_field(e3x_solver *h, int which, int rhs, const double *host)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!host) return fail(E3X_EINVAL, "host is NULL");
+    if (rhs < 0 || rhs >= s.nrhs) return fail(E3X_EINVAL, "rhs out of range");
+    double *dst = which == E3X_FIELD_U ? s.u : which == E3X_FIELD_GB ? s.g : which == E3X_FIELD_B ? s.b
+                  : which == E3X_FIELD_H ? s.h : nullptr;
+    if (!dst) return fail(E3X_EINVAL, "unknown field");
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(hipMemcpyAsync(dst + (int64_t)rhs * 3 * s.ns, host, sizeof(double) * 3 * (size_t)s.ns,
+                           hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    return E3X_OK;
+}
+
+int e3x_get_tables(e3x_solver *h, double *cmod, double *dk)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_femat) return fail(E3X_ESTATE, "e3x_get_tables before e3x_femat");
+    E3X_HIP(hipSetDevice(s.device));
+    if (cmod) E3X_HIP(hipMemcpyAsync(cmod, s.cmod, sizeof(double) * 36 * s.nphase, hipMemcpyDeviceToHost, s.stream));
+    if (dk) E3X_HIP(hipMemcpyAsync(dk, s.dk, sizeof(double) * 576 * s.nphase, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    return E3X_OK;
+}
+
+int e3x_last_timing(e3x_solver *h, double *precond_ms, double *pcg_ms)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (precond_ms) *precond_ms = s.last_precond_ms;
+    if (pcg_ms) *pcg_ms = s.last_pcg_ms;
+    return E3X_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Internal declarations of the gfx950 elas3dxtal_pcg solver (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace e3x {
+
+constexpr int kMaxRhs = 6;
+constexpr int kBlock = 256;        // threads per block of every kernel
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// Stencil tile: 8 x 8 x 4 nodes per block; each 64-lane wave owns a 4x4x4 sub-cube.
+constexpr int TX = 8, TY = 8, TZ = 4;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HN = HX * HY * HZ;  // node halo
+constexpr int VX = TX + 1, VY = TY + 1, VZ = TZ + 1, VN = VX * VY * VZ;  // voxel halo
+
+// Per right-hand-side PCG scalars, resident on the device (the locals of
+// dembx_OpenMP, f90:1440-1445).
+struct PcgState {
+    double gg, gg0, hah, lambda, gamma, rel_res, rel_res_prev, rel_res_min, tol;
+    int stop_iter;    // 0 while running, else the iteration at which the loop ended
+    int exit_flag;    // e3x_pcg_exit
+    int reset;        // this iteration reset h = z (f90:1961-1981)
+    int reset_count;
+    int run_dir;      // the stopping iteration still updates h (f90:2007 follows 1996)
+    int ldemb;
+    int pad[2];
+};
+
+enum StencilMode { kApply = 0, kEnergy = 1, kResid = 2, kPlain = 3 };
+
+struct Solver {
+    int nx = 0, ny = 0, nz = 0, nphase = 0, nrhs = 1, device = 0;
+    int64_t ns = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    int flag_m = 1;
+    double strain[6 * kMaxRhs] = {};
+
+    // phase tables
+    double *cmod = nullptr, *dk = nullptr, *sten = nullptr;  // [p][36], [p][576], [p][243]
+    float *dk32 = nullptr, *sten32 = nullptr;
+    double *orient = nullptr;                                  // [3*(nphase-1)]
+    uint16_t *pix = nullptr;                                   // 0-based phase per voxel
+
+    // fields, [nrhs][3][ns]
+    double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;
+    double *minv = nullptr;                                    // [6][ns]
+    float *g32 = nullptr, *h32 = nullptr, *z32 = nullptr, *ah32 = nullptr, *minv32 = nullptr;
+    double *cconst = nullptr;                                  // C per rhs
+    double *post = nullptr;                                    // full-field staging [7][ns]
+
+    // reductions and PCG state
+    double *partials = nullptr;
+    int64_t partials_cap = 0;
+    unsigned *counter = nullptr;
+    PcgState *st = nullptr;
+    PcgState *st_host = nullptr;     // pinned staging for the state upload / readback
+    double *hist = nullptr;
+    int64_t hist_cap = 0;
+    double *results = nullptr;     // small device result slots
+    double *results_host = nullptr;  // pinned mirror
+    int *flags_host = nullptr;       // pinned stop flags
+    unsigned *counts = nullptr;      // assig histogram
+
+    bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
+    double last_precond_ms = 0, last_pcg_ms = 0;
+    int64_t bytes = 0;
+};
+
+// ---- launchers (e3x_kernels.hip) ------------------------------------------
+hipError_t launch_femat(Solver &s, double c11, double c12, double c44, const double *phasemod4);
+hipError_t launch_rhs(Solver &s);
+hipError_t launch_affine(Solver &s);
+hipError_t launch_energy(Solver &s);                          // results[0..2*nrhs)
+hipError_t launch_precond(Solver &s, bool want32);
+hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
+                           const double *gg0_host);
+hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace);
+hipError_t launch_stress(Solver &s);                          // results[0..12*nrhs)
+hipError_t launch_fullfield(Solver &s, int rhs, double *stress_dev, double *vm_dev);
+hipError_t launch_assig(Solver &s);                           // counts[nphase]
+hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
+hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st);
+hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K x (FP64)
+
+int64_t stencil_blocks(const Solver &s);
+int64_t vector_blocks(const Solver &s);
+
+}  // namespace e3x

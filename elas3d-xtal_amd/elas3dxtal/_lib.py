@@ -1,0 +1,105 @@
+"""ctypes binding of libe3x.so (include/e3x.h).
+
+The HIP library is the only compute path: if it is missing this module raises, it
+never falls back to a CPU implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(os.path.dirname(PKG_DIR), "csrc")
+LIB_PATH = os.path.join(LIB_DIR, "libe3x.so")
+H5_LIB_PATH = os.path.join(LIB_DIR, "libe3x_h5.so")
+
+E3X_OK, E3X_EINVAL, E3X_EDEVICE, E3X_ENOMEM, E3X_EPHASE, E3X_ESTATE, E3X_EIO = range(7)
+PCG_RUNNING, PCG_CONVERGED, PCG_HAH_BREAK, PCG_MAXITER = range(4)
+FIELD_U, FIELD_GB, FIELD_B, FIELD_H = range(4)
+PREC_FP64, PREC_MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i32p = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+class E3xError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"e3x error {code}: {msg}")
+        self.code = code
+
+
+# name -> (restype, argtypes); mirrors include/e3x.h
+SIGNATURES = {
+    "e3x_last_error": (C.c_char_p, []),
+    "e3x_version": (C.c_int, []),
+    "e3x_device_count": (C.c_int, [_ip]),
+    "e3x_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "e3x_destroy": (C.c_int, [_vp]),
+    "e3x_memory_bytes": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "e3x_set_microstructure": (C.c_int, [_vp, _i32p, _dp]),
+    "e3x_assig": (C.c_int, [_vp, _dp]),
+    "e3x_femat": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _dp, C.c_int, _dp, _dp]),
+    "e3x_affine_guess": (C.c_int, [_vp]),
+    "e3x_energy": (C.c_int, [_vp, _dp, _dp]),
+    "e3x_dembx": (C.c_int, [_vp, _dp, C.c_float, C.c_int, C.c_int, _ip, _dp, _dp, _ip, _ip]),
+    "e3x_dembx_ex": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_float, C.c_int, C.c_int, _ip, _dp,
+                               _dp, _ip, _ip]),
+    "e3x_stress": (C.c_int, [_vp, _dp, _dp]),
+    "e3x_stress_fullfield": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "e3x_get_field": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "e3x_set_field": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "e3x_get_tables": (C.c_int, [_vp, _dp, _dp]),
+    "e3x_last_timing": (C.c_int, [_vp, _dp, _dp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libe3x.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(the HIP library is required; there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != E3X_OK:
+        msg = load().e3x_last_error().decode(errors="replace")
+        raise E3xError(rc, msg)
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+def i32ptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_i32p)
+
+
+def device_count():
+    n = C.c_int()
+    check(load().e3x_device_count(C.byref(n)))
+    return n.value

@@ -1,0 +1,175 @@
+/*
+ * e3x.h -- C ABI of the MI355X (gfx950) elas3dxtal_pcg solver.
+ *
+ * Drop-in boundary for the reference's solver routines in
+ * src/elas3dxtal_pcg.f90.  Each entry point below replaces one subroutine (cited)
+ * and is callable from Fortran through iso_c_binding (plain pointers, sizes and
+ * scalars; arrays in Fortran column-major layout, unchanged).  All device state
+ * lives in an opaque handle; NULL host pointers mean "keep it on the device".
+ *
+ * Conventions (all 0-based in C, identical memory layout to the Fortran arrays):
+ *   grid      nx*ny*nz periodic nodes, node m = i + nx*(j + ny*k)   (f90:350)
+ *   pix       int32[ns], phase id 1..nphase of the voxel at node m (f90:2945)
+ *   vectors   double[nrhs][3][ns]  == Fortran u(ns,3) per right-hand side
+ *   strain6   {exx, eyy, ezz, exz, eyz, exy} tensor components       (f90:394-399)
+ *   stress6   averages in order xx, yy, zz, xz, yz, xy; strains engineering
+ *             (f90:2379-2391)
+ *
+ * Every call returns E3X_OK (0) or a nonzero e3x_status; e3x_last_error() then
+ * holds a message (per thread).  Calls on one handle are synchronous and must
+ * come from one host thread at a time; different handles are independent.
+ */
+#ifndef E3X_H
+#define E3X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct e3x_solver e3x_solver;
+
+enum e3x_status {
+    E3X_OK = 0,
+    E3X_EINVAL = 1,   /* bad argument (sizes, NULL where required, ranges)          */
+    E3X_EDEVICE = 2,  /* HIP runtime error                                          */
+    E3X_ENOMEM = 3,   /* device allocation failed                                   */
+    E3X_EPHASE = 4,   /* pix label outside [1, nphase]   (replaces f90:2986-2989 stop) */
+    E3X_ESTATE = 5,   /* call order violated (e.g. dembx before femat)              */
+    E3X_EIO = 6       /* file I/O (HDF5 helpers)                                    */
+};
+
+/* How a PCG solve ended (dembx_OpenMP, f90:1899-1902, 2007, loop end). */
+enum e3x_pcg_exit {
+    E3X_PCG_RUNNING = 0,
+    E3X_PCG_CONVERGED = 1, /* rel_res < tol (f90:2007)                       */
+    E3X_PCG_HAH_BREAK = 2, /* |h.Ah| < 1e-14: WARNING and break (f90:1899)   */
+    E3X_PCG_MAXITER = 3    /* ldemb iterations without convergence          */
+};
+
+/* Fields reachable with e3x_get_field / e3x_set_field. */
+enum e3x_field {
+    E3X_FIELD_U = 0,  /* displacement u(ns,3)          (e3d%u)  */
+    E3X_FIELD_GB = 1, /* gradient gb = K u + b (ns,3)  (e3d%gb) */
+    E3X_FIELD_B = 2,  /* right-hand side b (ns,3)      (e3d%b)  */
+    E3X_FIELD_H = 3   /* search direction h (ns,3)     (e3d%h)  */
+};
+
+/* Arithmetic of e3x_dembx_ex. */
+enum e3x_precision {
+    E3X_PREC_FP64 = 0,  /* the reference's arithmetic                                   */
+    E3X_PREC_MIXED = 1  /* FP32 stencil, vectors and block-Jacobi; FP64 u, dots, norms
+                           and a true-residual replacement every `replace_every` steps */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *e3x_last_error(void);
+
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int e3x_version(void);
+
+/* Number of visible HIP devices. */
+int e3x_device_count(int *count);
+
+/*
+ * Create a solver for an nx*ny*nz periodic grid with nphase phases (grains plus
+ * the pore phase) and nrhs simultaneous right-hand sides (1..6; 6 = the unit
+ * load cases of the effective stiffness).  Replaces the allocations of
+ * program elas3d (f90:242-256).  nphase <= 65535.
+ */
+int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_solver **out);
+
+/* Free all device memory of the handle (f90:543-549). */
+int e3x_destroy(e3x_solver *s);
+
+/* Bytes of device memory the handle holds now. */
+int e3x_memory_bytes(const e3x_solver *s, int64_t *bytes);
+
+/*
+ * Load the microstructure: pix[ns] with labels in [1, nphase] (checked,
+ * E3X_EPHASE otherwise) and orientation[3*(nphase-1)] Rodrigues vectors
+ * tan(theta/2)*axis, grain-major.  Data part of ppixel_hdf5 (f90:2879-2991);
+ * the HDF5 file reading itself is in e3x_h5.h.  orientation may be NULL for
+ * flag_m = 0.
+ */
+int e3x_set_microstructure(e3x_solver *s, const int32_t *pix, const double *orientation);
+
+/* Phase volume fractions prob[nphase] (assig, f90:2410-2468). */
+int e3x_assig(e3x_solver *s, double *prob);
+
+/*
+ * femat_OpenMP (f90:555-1104): per-phase stiffness cmod (rotated cubic c11,c12,c44
+ * per grain for flag_m = 1, rotatestiffnessby f90:2470-2637; isotropic K,G for
+ * flag_m = 0), element blocks dk, the per-phase 27-point stencil blocks, and for
+ * each right-hand side r the periodic-jump force b_r and constant C_r from
+ * strain6[6*r .. 6*r+5].
+ *   phasemod: flag_m = 1 -> {K_pore, G_pore} (2 values, f90:301-317);
+ *             flag_m = 0 -> {K1, G1, K2, G2}  (f90:295-312, nphase must be 2).
+ *   C_out[nrhs] (may be NULL).
+ */
+int e3x_femat(e3x_solver *s, double c11, double c12, double c44, const double *phasemod,
+              int flag_m, const double *strain6, double *C_out);
+
+/* Affine initial guess u = E x for every right-hand side (f90:423-438). */
+int e3x_affine_guess(e3x_solver *s);
+
+/*
+ * energy_OpenMP (f90:1107-1353): gb = K u + b and utot = C + sum(u.Ku/2 + b.u)
+ * for every right-hand side; also gg[r] = sum(gb^2) (f90:452).  utot[nrhs],
+ * gg[nrhs] may be NULL.
+ */
+int e3x_energy(e3x_solver *s, double *utot, double *gg);
+
+/*
+ * dembx_OpenMP (f90:1356-2043): block-Jacobi PCG from the current u, gb.
+ *   gg0[nrhs]    initial sum(gb^2) per right-hand side (f90:452-453)
+ *   tol          single-precision tolerance on sqrt(z.g/gg0) (f90:148, 1949, 2007)
+ *   ldemb        iteration limit (f90:141)
+ *   kkk          outer step; kkk == 1 builds the preconditioner and sets h = z
+ *                (f90:1498, 1554); kkk > 1 keeps both (the reference reuses h and
+ *                leaves M_block_inv unset, f90:1469)
+ *   lstep[nrhs], gg[nrhs]           iterations taken, final z.g
+ *   history[nrhs*ldemb]             rel_res per iteration (f90:1952-1953, 2020); may be NULL
+ *   exit_flag[nrhs], reset_count[nrhs]  e3x_pcg_exit and divergence resets
+ *                                   (f90:1961-1981); may be NULL
+ */
+int e3x_dembx(e3x_solver *s, const double *gg0, float tol, int ldemb, int kkk, int *lstep,
+              double *gg, double *history, int *exit_flag, int *reset_count);
+
+/*
+ * e3x_dembx with a precision mode.  E3X_PREC_FP64 is e3x_dembx.  E3X_PREC_MIXED
+ * runs the stencil, vectors and preconditioner in FP32, keeps u, the dot products
+ * and norms in FP64, and every `replace_every` iterations replaces the recursive
+ * gradient by the true one, g = K u + b, computed in FP64 (reliable update).
+ * No counterpart in the reference (its solver is FP64 only).
+ */
+int e3x_dembx_ex(e3x_solver *s, int precision, int replace_every, const double *gg0, float tol,
+                 int ldemb, int kkk, int *lstep, double *gg, double *history, int *exit_flag,
+                 int *reset_count);
+
+/* stress_OpenMP (f90:2149-2408): stress6[6*nrhs], strain6[6*nrhs] volume averages. */
+int e3x_stress(e3x_solver *s, double *stress6, double *strain6);
+
+/*
+ * stress_fullfield_OpenMP computation part (f90:2639-2833) for right-hand side
+ * `rhs`: stress[6][ns] (Fortran stress_field(ns,6)) and von Mises vm[ns]; either
+ * may be NULL.  The HDF5 write (f90:2840-2874) is e3x_h5_write_fullfield.
+ */
+int e3x_stress_fullfield(e3x_solver *s, int rhs, double *stress, double *vm);
+
+/* Copy a field (enum e3x_field) of right-hand side `rhs` to / from host double[3*ns]. */
+int e3x_get_field(e3x_solver *s, int which, int rhs, double *host);
+int e3x_set_field(e3x_solver *s, int which, int rhs, const double *host);
+
+/* Copy the phase tables to the host: cmod[nphase*36] (cmod(p,i,j) row-major per
+ * phase) and dk[nphase*576] (dk(p,a,i,b,j) with j fastest); either may be NULL. */
+int e3x_get_tables(e3x_solver *s, double *cmod, double *dk);
+
+/* GPU time of the last e3x_dembx(_ex) call: preconditioner build and the PCG loop (ms). */
+int e3x_last_timing(e3x_solver *s, double *precond_ms, double *pcg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* E3X_H */

@@ -1,0 +1,31 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares (no GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from elas3dxtal import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(e3x_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_core_library_exports_header_symbols():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    names = declared("e3x.h")
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(L.SIGNATURES), "binding and header disagree"
+
+
+def test_version_and_error_string_without_gpu():
+    lib = L.load()
+    assert lib.e3x_version() >= 10000
+    assert isinstance(lib.e3x_last_error(), bytes)

@@ -1,0 +1,247 @@
+"""GPU parity of the HIP library (through the C ABI) against the CPU oracle.
+
+Tolerances: the FP64 path must match within 1e-10 relative on fields, energies and
+averages (every table and single-pass quantity within 1e-12); the iteration count
+of the reference's stop rule must match within 1 percent.
+"""
+import numpy as np
+import pytest
+
+import _indep as ind
+import _oracle as oc
+from elas3dxtal import _lib as L
+from elas3dxtal.solver import Elas3D, solve
+
+pytestmark = pytest.mark.gpu
+
+C11, C12, C44 = 246.5e9, 147.3e9, 124.7e9
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def voronoi_case(shape, ngrain, seed, pore_frac=0.02):
+    """Periodic Voronoi grains (nearest seed, minimum image) + random single-voxel pores."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    dims = np.array([nx, ny, nz], float)
+    seeds = rng.random((ngrain, 3)) * dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pts = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1) + 0.5
+    best = np.full(pts.shape[0], np.inf)
+    lab = np.zeros(pts.shape[0], np.int32)
+    for g in range(ngrain):
+        d = np.abs(pts - seeds[g])
+        d = np.minimum(d, dims - d)
+        d2 = np.sum(d * d, axis=1)
+        upd = d2 < best
+        best[upd] = d2[upd]
+        lab[upd] = g + 1
+    npore = int(round(pore_frac * pts.shape[0]))
+    if npore:
+        lab[rng.choice(pts.shape[0], npore, replace=False)] = ngrain + 1
+    rod = ind.random_rodrigues(rng, ngrain)
+    return lab, rod
+
+
+def oracle_tables(nphase, rod, pm=(0.0, 0.0)):
+    rot = oc.rotate_stiffness(rod, C11, C12, C44)
+    cm = oc.cmod(nphase, rot, list(pm))
+    return cm, oc.dk(cm)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    assert L.device_count() >= 1, "no HIP device visible"
+
+
+# ---------------------------------------------------------------- femat tables, b, C
+@pytest.mark.parametrize("shape,ngrain", [((4, 4, 4), 3), ((9, 7, 5), 6), ((16, 16, 16), 12)])
+def test_femat_tables_rhs_constant(shape, ngrain):
+    pix, rod = voronoi_case(shape, ngrain, seed=100 + ngrain)
+    nphase = ngrain + 1
+    e6 = [1.1e-3, -0.4e-3, 0.7e-3, 0.3e-3, -0.2e-3, 0.5e-3]
+    with Elas3D(shape, nphase) as s:
+        s.ppixel(pix, rod)
+        cval = s.femat(C11, C12, C44, e6)
+        cm, dk = s.tables()
+        b = s.get(L.FIELD_B)
+        prob = s.assig()
+    cm_o, dk_o = oracle_tables(nphase, rod)
+    assert _rel(cm, cm_o) < 1e-13
+    assert _rel(dk, dk_o) < 1e-13
+    b_o, c_o = oc.femat_rhs(shape, pix, dk_o, e6)
+    assert _rel(b, b_o) < 1e-12
+    assert abs(cval[0] - c_o) < 1e-12 * abs(c_o)
+    assert np.array_equal(prob, oc.assig(pix, nphase))          # integer counts: exact
+
+
+# ---------------------------------------------------------------- stencil + energy
+@pytest.mark.parametrize("shape,ngrain", [((4, 4, 4), 2), ((11, 6, 9), 5), ((24, 20, 16), 8)])
+def test_energy_and_stencil(shape, ngrain):
+    pix, rod = voronoi_case(shape, ngrain, seed=200 + ngrain, pore_frac=0.05)
+    nphase = ngrain + 1
+    ns = int(np.prod(shape))
+    e6 = [0.0, 0.0, oc.AML_F32, 0.2e-3, 0.0, -0.1e-3]
+    rng = np.random.default_rng(7)
+    u = oc.affine(shape, e6) + 1e-4 * rng.standard_normal(3 * ns)
+    cm_o, dk_o = oracle_tables(nphase, rod)
+    b_o, c_o = oc.femat_rhs(shape, pix, dk_o, e6)
+    gb_o, ut_o = oc.energy(shape, pix, dk_o, b_o, c_o, u)
+    with Elas3D(shape, nphase) as s:
+        s.ppixel(pix, rod)
+        s.femat(C11, C12, C44, e6)
+        s.set(L.FIELD_U, u)
+        ut, gg = s.energy()
+        gb = s.get(L.FIELD_GB)
+        # bare stencil: b = 0 gives gb = K x
+        x = rng.standard_normal(3 * ns)
+        s.set(L.FIELD_B, np.zeros(3 * ns))
+        s.set(L.FIELD_U, x)
+        s.energy()
+        kx = s.get(L.FIELD_GB)
+    assert _rel(gb, gb_o) < 1e-11
+    assert abs(ut[0] - ut_o) < 1e-12 * abs(ut_o)
+    assert abs(gg[0] - oc.sumsq(gb_o)) < 1e-10 * oc.sumsq(gb_o)
+    assert _rel(kx, oc.apply(shape, pix, dk_o, x)) < 1e-12
+
+
+def test_affine_guess_and_homogeneous_equilibrium():
+    shape = (10, 9, 8)
+    ns = int(np.prod(shape))
+    e6 = [0.9e-3, -0.3e-3, 0.2e-3, 0.4e-3, -0.1e-3, 0.25e-3]
+    with Elas3D(shape, 2) as s:
+        s.ppixel(np.ones(ns, np.int32), np.array([0.3, -0.2, 0.5]))
+        s.femat(C11, C12, C44, e6)
+        s.affine_guess()
+        u = s.get(L.FIELD_U)
+        ut, gg = s.energy()
+        cm, _ = s.tables()
+        gb = s.get(L.FIELD_GB)
+        b = s.get(L.FIELD_B)
+    assert np.array_equal(u, oc.affine(shape, e6))
+    assert np.max(np.abs(gb)) < 1e-10 * np.max(np.abs(b))
+    eps = np.array([e6[0], e6[1], e6[2], 2 * e6[3], 2 * e6[4], 2 * e6[5]])
+    assert abs(ut[0] - 0.5 * ns * eps @ cm[0] @ eps) < 1e-12 * abs(ut[0])
+
+
+# ---------------------------------------------------------------- PCG
+def _oracle_pcg(shape, pix, rod, e6, tol=oc.TOL_F32, ldemb=10000):
+    nphase = int(pix.max()) if rod is None else len(rod) + 1
+    cm_o, dk_o = oracle_tables(nphase, rod)
+    b_o, c_o = oc.femat_rhs(shape, pix, dk_o, e6)
+    u0 = oc.affine(shape, e6)
+    g0, ut0 = oc.energy(shape, pix, dk_o, b_o, c_o, u0)
+    gg0 = oc.sumsq(g0)
+    r = oc.dembx(shape, pix, dk_o, u0, g0, gg0, tol=tol, ldemb=ldemb)
+    return r, gg0
+
+
+@pytest.mark.parametrize("shape,ngrain", [((8, 8, 8), 4), ((13, 10, 7), 6), ((20, 20, 20), 10)])
+def test_dembx_matches_oracle(shape, ngrain):
+    pix, rod = voronoi_case(shape, ngrain, seed=300 + ngrain)
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    r_o, gg0_o = _oracle_pcg(shape, pix, rod, e6)
+    with Elas3D(shape, ngrain + 1) as s:
+        s.ppixel(pix, rod)
+        s.femat(C11, C12, C44, e6)
+        s.affine_guess()
+        ut0, gg0 = s.energy()
+        assert abs(gg0[0] - gg0_o) < 1e-11 * gg0_o
+        d = s.dembx(gg0)
+        u = s.get(L.FIELD_U)
+        gb = s.get(L.FIELD_GB)
+    assert d["exit"][0] == L.PCG_CONVERGED == r_o["status"]
+    assert abs(int(d["lstep"][0]) - r_o["lstep"]) <= max(1, int(0.01 * r_o["lstep"]))
+    if d["lstep"][0] == r_o["lstep"]:
+        assert _rel(u, r_o["u"]) < 1e-10
+        assert _rel(d["history"][0], r_o["history"]) < 1e-8
+        assert abs(d["gg"][0] - r_o["gg"]) < 1e-8 * r_o["gg"]
+
+
+def test_full_solve_matches_oracle_config1_shape():
+    """Plain FP64 PCG on a 50^3 periodic cube: 20 Voronoi grains, 2% single-voxel pores,
+    C11=246.5, C12=147.3, C44=124.7 GPa, exx = float32(1e-3)."""
+    shape = (50, 50, 50)
+    pix, rod = voronoi_case(shape, 20, seed=1000, pore_frac=0.02)
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    ro = oc.solve(shape, 21, pix, rod, C11, C12, C44, e6)
+    res, s = solve(pix, rod, shape, 21, C11, C12, C44, e6, keep=True)
+    try:
+        u = s.get(L.FIELD_U)
+        sf, vm = s.stress_fullfield()
+    finally:
+        s.close()
+    assert res.exit[0] == L.PCG_CONVERGED
+    assert abs(int(res.lstep[0]) - ro["lstep"]) <= max(1, int(0.01 * ro["lstep"]))
+    assert abs(res.utot[0] - ro["utot"]) < 1e-10 * abs(ro["utot"])
+    assert abs(res.utot0[0] - ro["utot0"]) < 1e-12 * abs(ro["utot0"])
+    assert _rel(res.stress6[0], ro["stress6"]) < 1e-10
+    assert _rel(res.strain6[0], ro["strain6"]) < 1e-10
+    assert _rel(u, ro["u"]) < 1e-10
+    cm_o, _ = oracle_tables(21, rod)
+    sf_o, vm_o = oc.fullfield(shape, pix, cm_o, e6, ro["u"])
+    assert _rel(sf, sf_o) < 1e-9
+    assert _rel(vm, vm_o) < 1e-9
+
+
+def test_hah_break_on_all_pore_grid():
+    shape = (4, 4, 4)
+    with Elas3D(shape, 2) as s:
+        s.ppixel(np.full(64, 2, np.int32), np.zeros(3))
+        s.femat(C11, C12, C44, [1e-3, 0, 0, 0, 0, 0])
+        s.affine_guess()
+        u0 = s.get(L.FIELD_U)
+        ut, gg0 = s.energy()
+        assert gg0[0] == 0.0
+        d = s.dembx(gg0)
+        assert d["exit"][0] == L.PCG_HAH_BREAK and d["lstep"][0] == 1
+        assert np.array_equal(s.get(L.FIELD_U), u0)
+
+
+def test_maxiter_and_history_prefix():
+    shape = (12, 12, 12)
+    pix, rod = voronoi_case(shape, 5, seed=400)
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    r_o, _ = _oracle_pcg(shape, pix, rod, e6, ldemb=5)
+    assert r_o["status"] == oc.OC_MAXITER
+    with Elas3D(shape, 6) as s:
+        s.ppixel(pix, rod)
+        s.femat(C11, C12, C44, e6)
+        s.affine_guess()
+        _, gg0 = s.energy()
+        d = s.dembx(gg0, ldemb=5)
+        u = s.get(L.FIELD_U)
+    assert d["exit"][0] == L.PCG_MAXITER and d["lstep"][0] == 5
+    assert _rel(d["history"][0], r_o["history"]) < 1e-10
+    assert _rel(u, r_o["u"]) < 1e-12
+
+
+def test_multi_rhs_equals_single_solves():
+    shape = (12, 10, 8)
+    pix, rod = voronoi_case(shape, 6, seed=500)
+    strains = np.array([[oc.AML_F32, 0, 0, 0, 0, 0], [0, 0, 0, 0.5e-3, 0, 0], [0, 0.7e-3, 0, 0, 0, 0.2e-3]])
+    batch = solve(pix, rod, shape, 7, C11, C12, C44, strains)
+    for r in range(3):
+        one = solve(pix, rod, shape, 7, C11, C12, C44, strains[r])
+        assert batch.lstep[r] == one.lstep[0]
+        assert abs(batch.utot[r] - one.utot[0]) <= 1e-13 * abs(one.utot[0])
+        assert _rel(batch.stress6[r], one.stress6[0]) < 1e-13
+
+
+def test_bad_phase_label_and_call_order():
+    with Elas3D((4, 4, 4), 3) as s:
+        with pytest.raises(L.E3xError) as ei:
+            s.energy()
+        assert ei.value.code == L.E3X_ESTATE
+        bad = np.ones(64, np.int32)
+        bad[5] = 4
+        with pytest.raises(L.E3xError) as ei:
+            s.ppixel(bad, np.zeros(6))
+        assert ei.value.code == L.E3X_EPHASE
+        bad[5] = 0
+        with pytest.raises(L.E3xError) as ei:
+            s.ppixel(bad, np.zeros(6))
+        assert ei.value.code == L.E3X_EPHASE
