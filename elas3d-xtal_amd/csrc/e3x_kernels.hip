@@ -344,6 +344,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *p
 __global__ __launch_bounds__(kBlock) void k_affine(int nx, int ny, int nz, int nrhs, int64_t ns,
                                                    JumpArgs A, double *u)
 {
+#pragma clang fp contract(off)
+    // separate multiply and add, as written in f90:432-434 (bitwise reproducible)
     const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (m >= ns) return;
     const double x = (double)(m % nx), y = (double)((m / nx) % ny), z = (double)(m / ((int64_t)nx * ny));

@@ -11,7 +11,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 LIB_PATH = os.path.join(LIB_DIR, "libe3x.so")
-H5_LIB_PATH = os.path.join(LIB_DIR, "libe3x_h5.so")
+IO_LIB_PATH = os.path.join(LIB_DIR, "libe3x_io.so")
 
 E3X_OK, E3X_EINVAL, E3X_EDEVICE, E3X_ENOMEM, E3X_EPHASE, E3X_ESTATE, E3X_EIO = range(7)
 PCG_RUNNING, PCG_CONVERGED, PCG_HAH_BREAK, PCG_MAXITER = range(4)
