@@ -1,0 +1,48 @@
+"""The Fortran driver (program elas3d over the C ABI) end to end on a generated HDF5
+input: its fullfield_poly.h5 and cgitr log agree with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import _oracle as oc
+from elas3dxtal import io
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "elas3d-xtal_amd", "fortran", "elas3d_gpu")
+C11, C12, C44 = 246.5e9, 147.3e9, 124.7e9
+
+
+def test_fortran_driver_matches_oracle(tmp_path):
+    assert os.path.exists(DRIVER), "build the driver first (__graft_entry__.build())"
+    shape, ngrain = (18, 16, 14), 9
+    pix, rod, _, _ = io.generate(shape, ngrain, 77, pore_mode=io.PORES_VOXELS, pore_fraction=0.02)
+    io.write_input(str(tmp_path / "input_structure_poly.h5"), shape, pix, rod)
+    (tmp_path / "elas3d.nml").write_text(
+        "&elas3d\n c11_local = 246.5d9, c12_local = 147.3d9, c44_local = 124.7d9,\n"
+        " aml_exx = 1.0000000474974513d-3, aml_ezz = 0.0d0\n/\n")
+    r = subprocess.run([DRIVER, "elas3d.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    ro = oc.solve(shape, ngrain + 1, pix, rod, C11, C12, C44, e6)
+    ns = int(np.prod(shape))
+    p2, vm, st = io.read_fullfield(str(tmp_path / "fullfield_poly.h5"), ns)
+    assert np.array_equal(p2, pix)
+    rot = oc.rotate_stiffness(rod, C11, C12, C44)
+    cm = oc.cmod(ngrain + 1, rot, [0.0, 0.0])
+    sf_o, vm_o = oc.fullfield(shape, pix, cm, e6, ro["u"])
+    assert np.max(np.abs(st - sf_o)) < 1e-9 * np.max(np.abs(sf_o))
+    assert np.max(np.abs(vm - vm_o)) < 1e-9 * np.max(np.abs(vm_o))
+    lines = (tmp_path / "cgitr_SS316L_polycrystal.txt").read_text().split("\n")
+    rows = [ln.split() for ln in lines if ln.strip()]
+    assert abs(len(rows) - ro["lstep"]) <= max(1, int(0.01 * ro["lstep"]))
+    if len(rows) == ro["lstep"]:
+        got = np.array([float(c[1]) for c in rows])
+        assert np.allclose(got, ro["history"], rtol=1e-7)
+    log = (tmp_path / "output_SS316L_polycrystal.txt").read_text()
+    assert "Number of conjugate steps=" in log and "Calculation complete!" in log
+    hist = (tmp_path / "cg_history_polycrystal.txt").read_text().split("\n")
+    assert len([h for h in hist if h.strip()]) == 2
