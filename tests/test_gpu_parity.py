@@ -177,7 +177,7 @@ def test_full_solve_matches_oracle_config1_shape():
     assert res.exit[0] == L.PCG_CONVERGED
     assert abs(int(res.lstep[0]) - ro["lstep"]) <= max(1, int(0.01 * ro["lstep"]))
     assert abs(res.utot[0] - ro["utot"]) < 1e-10 * abs(ro["utot"])
-    assert abs(res.utot0[0] - ro["utot0"]) < 1e-12 * abs(ro["utot0"])
+    assert abs(res.utot0[0] - ro["utot0"]) < 1e-10 * abs(ro["utot0"])
     assert _rel(res.stress6[0], ro["stress6"]) < 1e-10
     assert _rel(res.strain6[0], ro["strain6"]) < 1e-10
     assert _rel(u, ro["u"]) < 1e-10
