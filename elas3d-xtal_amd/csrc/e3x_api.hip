@@ -45,7 +45,7 @@ void release(Solver &s)
     (void)hipSetDevice(s.device);
     void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.u, s.g, s.b, s.h, s.z,
                    s.ah, s.minv, s.g32, s.h32, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
-                   s.counter, s.st, s.hist, s.results, s.counts, s.post};
+                   s.st, s.hist, s.results, s.post};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (s.results_host) (void)hipHostFree(s.results_host);
@@ -154,14 +154,13 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->ah, nv));
     CK(dalloc(*s, &s->minv, 6 * ns));
     CK(dalloc(*s, &s->cconst, kMaxRhs));
-    s->partials_cap = std::max<int64_t>(12 * (int64_t)nrhs * vector_blocks(*s), 2 * (int64_t)nrhs * stencil_blocks(*s));
-    s->partials_cap = std::max<int64_t>(s->partials_cap, (int64_t)kMaxRhs * vector_blocks(*s));
+    // partials: [values][blocks] of the largest reduction (stress: 12 per rhs over the
+    // streaming grid; energy: 2 per rhs over the stencil grid; rhs constant: 6)
+    s->partials_cap = std::max<int64_t>(12 * (int64_t)nrhs * vector_grid(*s), 2 * (int64_t)nrhs * stencil_blocks(*s));
+    s->partials_cap = std::max<int64_t>(s->partials_cap, (int64_t)kMaxRhs * vector_grid(*s));
     CK(dalloc(*s, &s->partials, s->partials_cap));
-    CK(dalloc(*s, &s->counter, 1));
-    CK(hipMemset(s->counter, 0, sizeof(unsigned)));
     CK(dalloc(*s, &s->st, kMaxRhs));
     CK(dalloc(*s, &s->results, 12 * kMaxRhs));
-    CK(dalloc(*s, &s->counts, nphase));
     CK(hipHostMalloc((void **)&s->results_host, sizeof(double) * 12 * kMaxRhs, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&s->flags_host, sizeof(int) * 2 * kMaxRhs, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&s->st_host, sizeof(PcgState) * 3 * kMaxRhs, hipHostMallocDefault));
@@ -206,7 +205,11 @@ int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orie
     if (lo < 1 || hi > s.nphase)
         return fail(E3X_EPHASE, "phase label in pix out of bounds: min " + std::to_string(lo) + ", max " +
                                     std::to_string(hi) + ", nphase " + std::to_string(s.nphase));
-    for (int64_t m = 0; m < s.ns; m++) p16[(size_t)m] = (uint16_t)(pix[m] - 1);
+    s.counts.assign((size_t)s.nphase, 0);
+    for (int64_t m = 0; m < s.ns; m++) {
+        p16[(size_t)m] = (uint16_t)(pix[m] - 1);
+        s.counts[(size_t)(pix[m] - 1)]++;
+    }
     E3X_HIP(hipSetDevice(s.device));
     E3X_HIP(hipMemcpyAsync(s.pix, p16.data(), sizeof(uint16_t) * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
     if (orientation && s.nphase > 1)
@@ -224,12 +227,8 @@ int e3x_assig(e3x_solver *h, double *prob)
     Solver &s = S(h);
     if (!prob) return fail(E3X_EINVAL, "prob is NULL");
     if (!s.have_pix) return fail(E3X_ESTATE, "e3x_assig before e3x_set_microstructure");
-    E3X_HIP(hipSetDevice(s.device));
-    E3X_HIP(launch_assig(s));
-    std::vector<unsigned> cnt((size_t)s.nphase);
-    E3X_HIP(hipMemcpyAsync(cnt.data(), s.counts, sizeof(unsigned) * s.nphase, hipMemcpyDeviceToHost, s.stream));
-    E3X_HIP(hipStreamSynchronize(s.stream));
-    for (int p = 0; p < s.nphase; p++) prob[p] = (double)cnt[(size_t)p] / (double)s.ns;
+    // f90:2455-2465: counts of each label over ns (exact integers, counted on upload)
+    for (int p = 0; p < s.nphase; p++) prob[p] = (double)s.counts[(size_t)p] / (double)s.ns;
     return E3X_OK;
 }
 

@@ -1,0 +1,82 @@
+// Device helpers shared by the gfx950 kernels (internal).
+#pragma once
+
+#include "e3x_internal.hpp"
+
+namespace e3x {
+
+// Local node a of the brick sits at (PX(a), PY(a), PZ(a)):
+// 1 (0,0,0) 2 (1,0,0) 3 (1,1,0) 4 (0,1,0) 5 (0,0,1) 6 (1,0,1) 7 (1,1,1) 8 (0,1,1)
+// (src/elas3dxtal_pcg.f90:639, 740-765).
+__host__ __device__ constexpr int PX(int a) { return ((a + 1) >> 1) & 1; }
+__host__ __device__ constexpr int PY(int a) { return (a >> 1) & 1; }
+__host__ __device__ constexpr int PZ(int a) { return a >> 2; }
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Consecutive tile ids on one XCD: the hardware hands workgroup b to XCD b % 8, so
+// neighbouring tiles (which share halo lines) land in the same L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb)
+{
+    const int xcd = bid % kXcds, k = bid / kXcds;
+    const int q = nb / kXcds, rem = nb % kXcds;
+    return xcd * q + (xcd < rem ? xcd : rem) + k;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Deterministic sum over the block (fixed shuffle tree per wave, then waves in order);
+// the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+    __shared__ double sh[NV][kBlock / 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        const double t = wave_sum(v[q]);
+        if (l == 0) sh[q][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double t = sh[q][0];
+            for (int ww = 1; ww < kBlock / 64; ww++) t += sh[q][ww];
+            v[q] = t;
+        }
+    }
+}
+
+// Block partials, layout [q][block]: plain stores, read by a separate finalize kernel
+// (no device-scope fences inside the producing kernel).
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double *partials)
+{
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) partials[(int64_t)q * gridDim.x + blockIdx.x] = v[q];
+    }
+}
+
+// In a single-block finalize kernel: fixed-order totals of the partials (thread 0).
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *partials, int nparts, double (&tot)[NV])
+{
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double t = 0.0;
+        for (int i = threadIdx.x; i < nparts; i += kBlock) t += partials[(int64_t)q * nparts + i];
+        tot[q] = t;
+    }
+    block_sum<NV>(tot);
+}
+
+}  // namespace e3x

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace e3x {
 
@@ -56,7 +57,6 @@ struct Solver {
     // reductions and PCG state
     double *partials = nullptr;
     int64_t partials_cap = 0;
-    unsigned *counter = nullptr;
     PcgState *st = nullptr;
     PcgState *st_host = nullptr;     // pinned staging for the state upload / readback
     double *hist = nullptr;
@@ -64,30 +64,39 @@ struct Solver {
     double *results = nullptr;     // small device result slots
     double *results_host = nullptr;  // pinned mirror
     int *flags_host = nullptr;       // pinned stop flags
-    unsigned *counts = nullptr;      // assig histogram
+    std::vector<int64_t> counts;     // phase histogram of pix (assig)
 
     bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
     double last_precond_ms = 0, last_pcg_ms = 0;
     int64_t bytes = 0;
 };
 
-// ---- launchers (e3x_kernels.hip) ------------------------------------------
+// ---- launchers --------------------------------------------------------------
+// e3x_setup.hip
 hipError_t launch_femat(Solver &s, double c11, double c12, double c44, const double *phasemod4);
-hipError_t launch_rhs(Solver &s);
+hipError_t launch_rhs(Solver &s);                             // b and cconst
 hipError_t launch_affine(Solver &s);
-hipError_t launch_energy(Solver &s);                          // results[0..2*nrhs)
 hipError_t launch_precond(Solver &s, bool want32);
+// e3x_stencil.hip
+hipError_t launch_energy(Solver &s);                          // results[0..2*nrhs)
+hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K x (FP64)
+hipError_t launch_apply_hah(Solver &s, bool mixed, int it);   // Ah = K h, hAh, lambda
+hipError_t launch_resid(Solver &s);                           // g32 = K u + b in FP64
+// e3x_pcg.hip
+hipError_t launch_hah_finish(Solver &s, int nparts, int it);
 hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
                            const double *gg0_host);
 hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace);
+// e3x_post.hip
 hipError_t launch_stress(Solver &s);                          // results[0..12*nrhs)
 hipError_t launch_fullfield(Solver &s, int rhs, double *stress_dev, double *vm_dev);
-hipError_t launch_assig(Solver &s);                           // counts[nphase]
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
 hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st);
-hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K x (FP64)
 
 int64_t stencil_blocks(const Solver &s);
-int64_t vector_blocks(const Solver &s);
+inline int64_t vector_blocks(const Solver &s) { return (s.ns + kBlock - 1) / kBlock; }
+// streaming kernels: grid-stride with at most kVecGridMax blocks (8 waves per SIMD)
+constexpr int kVecGridMax = 2048;
+inline int64_t vector_grid(const Solver &s) { return vector_blocks(s) < kVecGridMax ? vector_blocks(s) : kVecGridMax; }
 
 }  // namespace e3x

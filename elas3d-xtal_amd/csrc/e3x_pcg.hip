@@ -1,0 +1,353 @@
+// gfx950 PCG loop of dembx_OpenMP (src/elas3dxtal_pcg.f90:1356-2043).
+//
+// One iteration = five launches on the solver's stream, all gated on device-resident
+// per-rhs stop state (so the host can enqueue iterations ahead and stop launching
+// when it sees every right-hand side finished):
+//   stencil  Ah = A h, block partials of h.Ah        (e3x_stencil.hip)
+//   fin_hah  hAh, |hAh| < 1e-14 break, lambda         (f90:1889-1907)
+//   update   u -= lambda h, g -= lambda Ah, z = M^-1 g, partials of z.g (f90:1914-1946)
+//   fin_upd  gg, rel_res, history, reset / gamma, stop test (f90:1949-2007)
+//   direction h = z (+ gamma h)                      (f90:1961-1996)
+// Storage type T of g, h, z, Ah and M^-1 is double, or float in the mixed form; u and
+// every dot product stay double.
+#include "e3x_device.hpp"
+
+#include <cmath>
+
+namespace e3x {
+namespace {
+
+template <typename T>
+struct PcgArgs {
+    int64_t ns;
+    int it, ldemb, kkk;
+    double *u;
+    T *g, *h, *z, *ah;
+    const T *minv;
+    double *partials;
+    PcgState *st;
+    double *hist;
+};
+
+template <typename T>
+__device__ __forceinline__ void precond_apply(const T *minv, int64_t ns, int64_t m, double g0, double g1,
+                                              double g2, double &z0, double &z1, double &z2)
+{
+    const double a00 = minv[m], a11 = minv[ns + m], a22 = minv[2 * ns + m];
+    const double a01 = minv[3 * ns + m], a02 = minv[4 * ns + m], a12 = minv[5 * ns + m];
+    z0 = a00 * g0 + a01 * g1 + a02 * g2;
+    z1 = a01 * g0 + a11 * g1 + a12 * g2;
+    z2 = a02 * g0 + a12 * g1 + a22 * g2;
+}
+
+// z = M^-1 g, partials of z.g, and (kkk == 1) h = z  (f90:1527-1562)
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs<T> A)
+{
+    const int64_t ns = A.ns;
+    double part[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) part[r] = 0.0;
+    for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            T *g = A.g + (int64_t)r * 3 * ns, *z = A.z + (int64_t)r * 3 * ns, *h = A.h + (int64_t)r * 3 * ns;
+            const double g0 = g[m], g1 = g[ns + m], g2 = g[2 * ns + m];
+            double z0, z1, z2;
+            precond_apply(A.minv, ns, m, g0, g1, g2, z0, z1, z2);
+            const T t0 = (T)z0, t1 = (T)z1, t2 = (T)z2;
+            z[m] = t0; z[ns + m] = t1; z[2 * ns + m] = t2;
+            if (A.kkk == 1) { h[m] = t0; h[ns + m] = t1; h[2 * ns + m] = t2; }
+            part[r] += (double)t0 * g0 + (double)t1 * g1 + (double)t2 * g2;
+        }
+    }
+    write_partials<NR>(part, A.partials);
+}
+
+// loop-entry scalars (f90:1564-1567)
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_pcg_init_finish(const double *partials, int nparts, PcgState *st,
+                                                            int ldemb)
+{
+    double tot[NR];
+    sum_partials<NR>(partials, nparts, tot);
+    if (threadIdx.x != 0) return;
+    for (int r = 0; r < NR; r++) {
+        PcgState &S = st[r];
+        S.gg = tot[r];
+        S.rel_res_prev = sqrt(S.gg / S.gg0);
+        S.rel_res_min = S.rel_res_prev;
+        S.rel_res = S.rel_res_prev;
+        S.reset_count = 0;
+        S.reset = 0;
+        S.run_dir = 0;
+        S.stop_iter = ldemb <= 0 ? -1 : 0;
+        S.exit_flag = ldemb <= 0 ? 3 : 0;
+    }
+}
+
+// hAh, the curvature guard and lambda (f90:1889-1907)
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_hah_finish(const double *partials, int nparts, PcgState *st, int it)
+{
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) any |= st[r].stop_iter == 0;
+    if (!any) return;
+    double tot[NR];
+    sum_partials<NR>(partials, nparts, tot);
+    if (threadIdx.x != 0) return;
+    for (int r = 0; r < NR; r++) {
+        PcgState &S = st[r];
+        if (S.stop_iter != 0) continue;
+        S.hah = tot[r];
+        if (fabs(tot[r]) < 1e-14) {
+            S.stop_iter = it;
+            S.exit_flag = 2;  // E3X_PCG_HAH_BREAK
+            S.run_dir = 0;
+        } else {
+            S.lambda = S.gg / tot[r];
+        }
+    }
+}
+
+// u -= lambda h; g -= lambda Ah (unless the true gradient replaces g this iteration);
+// z = M^-1 g; partials of z.g   (f90:1914-1946)
+template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
+{
+    bool act[NR];
+    double lam[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        act[r] = A.st[r].stop_iter == 0;
+        lam[r] = A.st[r].lambda;
+        any |= act[r];
+    }
+    if (!any) return;
+    const int64_t ns = A.ns;
+    double part[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) part[r] = 0.0;
+    for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (!act[r]) continue;
+            const int64_t o = (int64_t)r * 3 * ns + m;
+            if constexpr (DO_U) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) A.u[o + c * ns] = A.u[o + c * ns] - lam[r] * (double)A.h[o + c * ns];
+            }
+            double g[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                g[c] = (double)A.g[o + c * ns];
+                if constexpr (UPDATE_G) {
+                    const T gn = (T)(g[c] - lam[r] * (double)A.ah[o + c * ns]);
+                    A.g[o + c * ns] = gn;
+                    g[c] = (double)gn;
+                }
+            }
+            if constexpr (DO_Z) {
+                double z0, z1, z2;
+                precond_apply(A.minv, ns, m, g[0], g[1], g[2], z0, z1, z2);
+                const T t0 = (T)z0, t1 = (T)z1, t2 = (T)z2;
+                A.z[o] = t0; A.z[o + ns] = t1; A.z[o + 2 * ns] = t2;
+                part[r] += (double)t0 * g[0] + (double)t1 * g[1] + (double)t2 * g[2];
+            }
+        }
+    }
+    if constexpr (DO_Z) write_partials<NR>(part, A.partials);
+}
+
+// gg, rel_res, history, divergence reset or gamma, stop test (f90:1938-2007)
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_update_finish(const double *partials, int nparts, PcgState *st,
+                                                          double *hist, int it)
+{
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) any |= st[r].stop_iter == 0;
+    if (!any) return;
+    double tot[NR];
+    sum_partials<NR>(partials, nparts, tot);
+    if (threadIdx.x != 0) return;
+    for (int r = 0; r < NR; r++) {
+        PcgState &S = st[r];
+        if (S.stop_iter != 0) continue;
+        const double gglast = S.gg;
+        S.gg = tot[r];
+        const double rel = sqrt(S.gg / S.gg0);
+        S.rel_res = rel;
+        hist[(int64_t)r * S.ldemb + it - 1] = rel;
+        if (rel > 10.0 * S.rel_res_prev && it > 1) {
+            S.reset = 1;
+            S.rel_res_min = rel;
+            S.reset_count += 1;
+            S.gamma = 0.0;
+        } else {
+            S.reset = 0;
+            S.gamma = S.gg / gglast;
+            if (rel < S.rel_res_min) S.rel_res_min = rel;
+        }
+        S.rel_res_prev = rel;
+        if (rel < S.tol) {
+            S.stop_iter = it;
+            S.exit_flag = 1;  // E3X_PCG_CONVERGED
+            S.run_dir = 1;
+        } else if (it >= S.ldemb) {
+            S.stop_iter = it;
+            S.exit_flag = 3;  // E3X_PCG_MAXITER
+            S.run_dir = 1;
+        }
+    }
+}
+
+// h = z + gamma h, or h = z on a reset (f90:1961-1996)
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void k_pcg_direction(PcgArgs<T> A)
+{
+    bool run[NR], rst[NR];
+    double gam[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const PcgState &S = A.st[r];
+        run[r] = S.stop_iter == 0 || (S.stop_iter == A.it && S.run_dir);
+        rst[r] = S.reset != 0;
+        gam[r] = S.gamma;
+        any |= run[r];
+    }
+    if (!any) return;
+    const int64_t ns = A.ns;
+    for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (!run[r]) continue;
+            const int64_t o = (int64_t)r * 3 * ns + m;
+            if (rst[r]) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) A.h[o + c * ns] = A.z[o + c * ns];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    A.h[o + c * ns] = (T)((double)A.z[o + c * ns] + gam[r] * (double)A.h[o + c * ns]);
+            }
+        }
+    }
+}
+
+template <typename T>
+PcgArgs<T> pcg_args(const Solver &s, int it)
+{
+    PcgArgs<T> A{};
+    A.ns = s.ns; A.it = it; A.u = s.u;
+    A.partials = s.partials; A.st = s.st; A.hist = s.hist;
+    if constexpr (sizeof(T) == 4) {
+        A.g = s.g32; A.h = s.h32; A.z = s.z32; A.ah = s.ah32; A.minv = s.minv32;
+    } else {
+        A.g = s.g; A.h = s.h; A.z = s.z; A.ah = s.ah; A.minv = s.minv;
+    }
+    return A;
+}
+
+template <typename T, int NR>
+void init_nr(Solver &s, int kkk, int ldemb)
+{
+    const int g = (int)vector_grid(s);
+    PcgArgs<T> A = pcg_args<T>(s, 0);
+    A.kkk = kkk; A.ldemb = ldemb;
+    hipLaunchKernelGGL((k_pcg_init<T, NR>), dim3(g), dim3(kBlock), 0, s.stream, A);
+    hipLaunchKernelGGL(k_pcg_init_finish<NR>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, g,
+                       s.st, ldemb);
+}
+
+template <typename T, int NR>
+void iter_nr(Solver &s, int it, bool replace)
+{
+    const int g = (int)vector_grid(s);
+    const dim3 vg(g), blk(kBlock), one(1);
+    // 1. Ah = A h and the curvature (f90:1578-1907)
+    (void)launch_apply_hah(s, sizeof(T) == 4, it);
+    PcgArgs<T> A = pcg_args<T>(s, it);
+    // 2. updates, z and z.g (f90:1914-2007)
+    if (!replace) {
+        hipLaunchKernelGGL((k_pcg_update<T, NR, true, true, true>), vg, blk, 0, s.stream, A);
+    } else {
+        // reliable update: u first, then the true gradient g = K u + b in FP64, then z
+        hipLaunchKernelGGL((k_pcg_update<T, NR, false, true, false>), vg, blk, 0, s.stream, A);
+        (void)launch_resid(s);
+        hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true>), vg, blk, 0, s.stream, A);
+    }
+    hipLaunchKernelGGL(k_update_finish<NR>, one, blk, 0, s.stream, (const double *)s.partials, g, s.st, s.hist, it);
+    // 3. direction (f90:1961-1996)
+    hipLaunchKernelGGL((k_pcg_direction<T, NR>), vg, blk, 0, s.stream, A);
+}
+
+template <typename T>
+void init_t(Solver &s, int kkk, int ldemb)
+{
+    switch (s.nrhs) {
+    case 1: init_nr<T, 1>(s, kkk, ldemb); break;
+    case 2: init_nr<T, 2>(s, kkk, ldemb); break;
+    case 3: init_nr<T, 3>(s, kkk, ldemb); break;
+    case 4: init_nr<T, 4>(s, kkk, ldemb); break;
+    case 5: init_nr<T, 5>(s, kkk, ldemb); break;
+    default: init_nr<T, 6>(s, kkk, ldemb); break;
+    }
+}
+
+template <typename T>
+void iter_t(Solver &s, int it, bool replace)
+{
+    switch (s.nrhs) {
+    case 1: iter_nr<T, 1>(s, it, replace); break;
+    case 2: iter_nr<T, 2>(s, it, replace); break;
+    case 3: iter_nr<T, 3>(s, it, replace); break;
+    case 4: iter_nr<T, 4>(s, it, replace); break;
+    case 5: iter_nr<T, 5>(s, it, replace); break;
+    default: iter_nr<T, 6>(s, it, replace); break;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_hah_finish(Solver &s, int nparts, int it)
+{
+    switch (s.nrhs) {
+    case 1: hipLaunchKernelGGL(k_hah_finish<1>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 2: hipLaunchKernelGGL(k_hah_finish<2>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 3: hipLaunchKernelGGL(k_hah_finish<3>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 4: hipLaunchKernelGGL(k_hah_finish<4>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 5: hipLaunchKernelGGL(k_hah_finish<5>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    default: hipLaunchKernelGGL(k_hah_finish<6>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol, const double *gg0)
+{
+    // state scalars from the host (pinned staging): gg0, tol, ldemb
+    PcgState *hs = s.st_host;
+    for (int r = 0; r < s.nrhs; r++) {
+        hs[r] = PcgState{};
+        hs[r].gg0 = gg0[r];
+        hs[r].tol = (double)tol;
+        hs[r].ldemb = ldemb;
+    }
+    hipError_t e = hipMemcpyAsync(s.st, hs, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream);
+    if (e != hipSuccess) return e;
+    if (mixed) init_t<float>(s, kkk, ldemb);
+    else init_t<double>(s, kkk, ldemb);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace)
+{
+    if (mixed) iter_t<float>(s, it, replace);
+    else iter_t<double>(s, it, false);
+    return hipGetLastError();
+}
+
+}  // namespace e3x

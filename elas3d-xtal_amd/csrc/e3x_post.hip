@@ -1,0 +1,166 @@
+// gfx950 post-processing kernels of the elas3dxtal_pcg solve: volume-averaged and
+// full-field voxel-centre stress/strain, and precision conversions.
+#include "e3x_device.hpp"
+
+#include <cmath>
+
+namespace e3x {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Voxel-centre strain and stress (stress_OpenMP f90:2149-2408,
+// stress_fullfield_OpenMP f90:2639-2833).  es at the centre is +-1/4.
+struct PostArgs {
+    int nx, ny, nz, nrhs;
+    int64_t ns;
+    const uint16_t *pix;
+    const double *cmod, *u;
+    double e[6 * kMaxRhs];
+};
+
+__device__ void voxel_strain_stress(const PostArgs &A, int r, int64_t m, double eps[6], double sig[6])
+{
+    const int i = (int)(m % A.nx), j = (int)((m / A.nx) % A.ny), k = (int)(m / ((int64_t)A.nx * A.ny));
+    const double *u = A.u + (int64_t)r * 3 * A.ns;
+    const double *e = A.e + 6 * r;  // exx eyy ezz exz eyz exy
+    double uu[8][3];
+    for (int a = 0; a < 8; a++) {
+        const int64_t q = wrap(i + PX(a), A.nx) + (int64_t)A.nx * (wrap(j + PY(a), A.ny) + (int64_t)A.ny * wrap(k + PZ(a), A.nz));
+        for (int c = 0; c < 3; c++) uu[a][c] = u[c * A.ns + q];
+    }
+    if (i == A.nx - 1)
+        for (int a = 0; a < 8; a++)
+            if (PX(a)) { uu[a][0] += e[0] * A.nx; uu[a][1] += e[5] * A.nx; uu[a][2] += e[3] * A.nx; }
+    if (j == A.ny - 1)
+        for (int a = 0; a < 8; a++)
+            if (PY(a)) { uu[a][0] += e[5] * A.ny; uu[a][1] += e[1] * A.ny; uu[a][2] += e[4] * A.ny; }
+    if (k == A.nz - 1)
+        for (int a = 0; a < 8; a++)
+            if (PZ(a)) { uu[a][0] += e[3] * A.nz; uu[a][1] += e[4] * A.nz; uu[a][2] += e[2] * A.nz; }
+    for (int q = 0; q < 6; q++) eps[q] = 0.0;
+    for (int n3 = 0; n3 < 3; n3++)
+        for (int a = 0; a < 8; a++) {
+            const double dx = PX(a) ? 0.25 : -0.25, dy = PY(a) ? 0.25 : -0.25, dz = PZ(a) ? 0.25 : -0.25;
+            const double v = uu[a][n3];
+            if (n3 == 0) { eps[0] += dx * v; eps[3] += dz * v; eps[5] += dy * v; }
+            else if (n3 == 1) { eps[1] += dy * v; eps[4] += dz * v; eps[5] += dx * v; }
+            else { eps[2] += dz * v; eps[3] += dx * v; eps[4] += dy * v; }
+        }
+    const double *c = A.cmod + (int64_t)A.pix[m] * 36;
+    for (int q = 0; q < 6; q++) {
+        double s = 0.0;
+        for (int t = 0; t < 6; t++) s += c[q * 6 + t] * eps[t];
+        sig[q] = s;
+    }
+}
+
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_stress(PostArgs A, double *partials)
+{
+    double part[12 * NR];
+#pragma unroll
+    for (int q = 0; q < 12 * NR; q++) part[q] = 0.0;
+    for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < A.ns; m += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            double eps[6], sig[6];
+            voxel_strain_stress(A, r, m, eps, sig);
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                part[12 * r + q] += sig[q];
+                part[12 * r + 6 + q] += eps[q];
+            }
+        }
+    }
+    write_partials<12 * NR>(part, partials);
+}
+
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_stress_finish(const double *partials, int nparts, int64_t ns,
+                                                          double *results)
+{
+    double tot[12 * NR];
+    sum_partials<12 * NR>(partials, nparts, tot);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 12 * NR; q++) results[q] = tot[q] / (double)ns;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fullfield(PostArgs A, int r, double *sf, double *vm)
+{
+    const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (m >= A.ns) return;
+    double eps[6], s[6];
+    voxel_strain_stress(A, r, m, eps, s);
+    if (sf)
+        for (int q = 0; q < 6; q++) sf[(int64_t)q * A.ns + m] = s[q];
+    if (vm)
+        vm[m] = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) +
+                            (s[2] - s[0]) * (s[2] - s[0]) + 6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+}
+
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(const double *s, float *d, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f64(const float *s, double *d, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) d[i] = (double)s[i];
+}
+
+PostArgs post_args(const Solver &s)
+{
+    PostArgs A{};
+    A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.nrhs = s.nrhs; A.ns = s.ns;
+    A.pix = s.pix; A.cmod = s.cmod; A.u = s.u;
+    for (int q = 0; q < 6 * kMaxRhs; q++) A.e[q] = s.strain[q];
+    return A;
+}
+
+template <int NR>
+void stress_nr(Solver &s)
+{
+    const int g = (int)vector_grid(s);
+    hipLaunchKernelGGL(k_stress<NR>, dim3(g), dim3(kBlock), 0, s.stream, post_args(s), s.partials);
+    hipLaunchKernelGGL(k_stress_finish<NR>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, g, s.ns,
+                       s.results);
+}
+
+}  // namespace
+
+hipError_t launch_stress(Solver &s)
+{
+    switch (s.nrhs) {
+    case 1: stress_nr<1>(s); break;
+    case 2: stress_nr<2>(s); break;
+    case 3: stress_nr<3>(s); break;
+    case 4: stress_nr<4>(s); break;
+    case 5: stress_nr<5>(s); break;
+    default: stress_nr<6>(s); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fullfield(Solver &s, int rhs, double *sf, double *vm)
+{
+    hipLaunchKernelGGL(k_fullfield, dim3((unsigned)vector_blocks(s)), dim3(kBlock), 0, s.stream,
+                       post_args(s), rhs, sf, vm);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_f64, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace e3x
