@@ -30,12 +30,12 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Deterministic sum over the block (fixed shuffle tree per wave, then waves in order);
-// the result is valid in thread 0.
-template <int NV>
+// Deterministic sum over a block of NT threads (fixed shuffle tree per wave, then the
+// waves in order); the result is valid in thread 0.
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV])
 {
-    __shared__ double sh[NV][kBlock / 64];
+    __shared__ double sh[NV][NT / 64];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     __syncthreads();
 #pragma unroll
@@ -48,7 +48,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
 #pragma unroll
         for (int q = 0; q < NV; q++) {
             double t = sh[q][0];
-            for (int ww = 1; ww < kBlock / 64; ww++) t += sh[q][ww];
+            for (int ww = 1; ww < NT / 64; ww++) t += sh[q][ww];
             v[q] = t;
         }
     }
@@ -66,17 +66,28 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *partials
     }
 }
 
-// In a single-block finalize kernel: fixed-order totals of the partials (thread 0).
+// Threads of the single-block finalize kernels.
+constexpr int kFinBlock = 1024;
+
+// In a finalize kernel of kFinBlock threads: fixed-order totals of the partials, valid
+// in thread 0.  Each thread keeps 8 independent loads in flight.
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double *partials, int nparts, double (&tot)[NV])
 {
 #pragma unroll
     for (int q = 0; q < NV; q++) {
-        double t = 0.0;
-        for (int i = threadIdx.x; i < nparts; i += kBlock) t += partials[(int64_t)q * nparts + i];
-        tot[q] = t;
+        const double *p = partials + (int64_t)q * nparts;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = threadIdx.x; i < nparts; i += 8 * kFinBlock) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = i + u * kFinBlock;
+                if (idx < nparts) acc[u] += p[idx];
+            }
+        }
+        tot[q] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     }
-    block_sum<NV>(tot);
+    block_sum<NV, kFinBlock>(tot);
 }
 
 }  // namespace e3x

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs<T> A)
 
 // loop-entry scalars (f90:1564-1567)
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_pcg_init_finish(const double *partials, int nparts, PcgState *st,
+__global__ __launch_bounds__(kFinBlock) void k_pcg_init_finish(const double *partials, int nparts, PcgState *st,
                                                             int ldemb)
 {
     double tot[NR];
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_finish(const double *partia
 
 // hAh, the curvature guard and lambda (f90:1889-1907)
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_hah_finish(const double *partials, int nparts, PcgState *st, int it)
+__global__ __launch_bounds__(kFinBlock) void k_hah_finish(const double *partials, int nparts, PcgState *st, int it)
 {
     bool any = false;
 #pragma unroll
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
 
 // gg, rel_res, history, divergence reset or gamma, stop test (f90:1938-2007)
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_update_finish(const double *partials, int nparts, PcgState *st,
+__global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *partials, int nparts, PcgState *st,
                                                           double *hist, int it)
 {
     bool any = false;
@@ -259,7 +259,7 @@ void init_nr(Solver &s, int kkk, int ldemb)
     PcgArgs<T> A = pcg_args<T>(s, 0);
     A.kkk = kkk; A.ldemb = ldemb;
     hipLaunchKernelGGL((k_pcg_init<T, NR>), dim3(g), dim3(kBlock), 0, s.stream, A);
-    hipLaunchKernelGGL(k_pcg_init_finish<NR>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, g,
+    hipLaunchKernelGGL(k_pcg_init_finish<NR>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g,
                        s.st, ldemb);
 }
 
@@ -280,7 +280,7 @@ void iter_nr(Solver &s, int it, bool replace)
         (void)launch_resid(s);
         hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true>), vg, blk, 0, s.stream, A);
     }
-    hipLaunchKernelGGL(k_update_finish<NR>, one, blk, 0, s.stream, (const double *)s.partials, g, s.st, s.hist, it);
+    hipLaunchKernelGGL(k_update_finish<NR>, one, dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.st, s.hist, it);
     // 3. direction (f90:1961-1996)
     hipLaunchKernelGGL((k_pcg_direction<T, NR>), vg, blk, 0, s.stream, A);
 }
@@ -316,12 +316,12 @@ void iter_t(Solver &s, int it, bool replace)
 hipError_t launch_hah_finish(Solver &s, int nparts, int it)
 {
     switch (s.nrhs) {
-    case 1: hipLaunchKernelGGL(k_hah_finish<1>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 2: hipLaunchKernelGGL(k_hah_finish<2>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 3: hipLaunchKernelGGL(k_hah_finish<3>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 4: hipLaunchKernelGGL(k_hah_finish<4>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 5: hipLaunchKernelGGL(k_hah_finish<5>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    default: hipLaunchKernelGGL(k_hah_finish<6>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 1: hipLaunchKernelGGL(k_hah_finish<1>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 2: hipLaunchKernelGGL(k_hah_finish<2>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 3: hipLaunchKernelGGL(k_hah_finish<3>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 4: hipLaunchKernelGGL(k_hah_finish<4>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    case 5: hipLaunchKernelGGL(k_hah_finish<5>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
+    default: hipLaunchKernelGGL(k_hah_finish<6>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
     }
     return hipGetLastError();
 }

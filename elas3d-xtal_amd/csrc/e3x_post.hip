@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(PostArgs A, double *partials)
 }
 
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_stress_finish(const double *partials, int nparts, int64_t ns,
+__global__ __launch_bounds__(kFinBlock) void k_stress_finish(const double *partials, int nparts, int64_t ns,
                                                           double *results)
 {
     double tot[12 * NR];
@@ -125,7 +125,7 @@ void stress_nr(Solver &s)
 {
     const int g = (int)vector_grid(s);
     hipLaunchKernelGGL(k_stress<NR>, dim3(g), dim3(kBlock), 0, s.stream, post_args(s), s.partials);
-    hipLaunchKernelGGL(k_stress_finish<NR>, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, g, s.ns,
+    hipLaunchKernelGGL(k_stress_finish<NR>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.ns,
                        s.results);
 }
 

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *p
     write_partials<kMaxRhs>(cpart, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_rhs_finish(const double *partials, int nparts, int nrhs,
+__global__ __launch_bounds__(kFinBlock) void k_rhs_finish(const double *partials, int nparts, int nrhs,
                                                        double *cconst)
 {
     double tot[kMaxRhs];
@@ -334,7 +334,7 @@ hipError_t launch_rhs(Solver &s)
 {
     const int g = (int)vector_grid(s);
     hipLaunchKernelGGL(k_rhs, dim3(g), dim3(kBlock), 0, s.stream, jump_args(s), s.b, s.partials);
-    hipLaunchKernelGGL(k_rhs_finish, dim3(1), dim3(kBlock), 0, s.stream, (const double *)s.partials, g, s.nrhs,
+    hipLaunchKernelGGL(k_rhs_finish, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.nrhs,
                        s.cconst);
     return hipGetLastError();
 }

@@ -10,7 +10,8 @@
 //   - to its homogeneous lanes of phase P the 27-point stencil of P (243 FMA),
 //   - to its mixed lanes, for every octant a whose voxel has phase P, the 3x24 rows of
 //     the element matrix dk[P][a] (72 FMA).
-// The 81 neighbour values of a lane are read once from LDS into registers.
+// Neighbour values are read from LDS at their use; registers stay below 128 so four
+// or more waves per SIMD hide the scalar-cache latency of the coefficient loads.
 #include "e3x_device.hpp"
 
 #include <cmath>
@@ -32,8 +33,11 @@ struct StencilArgs {
     const PcgState *st;    // kApply, kResid: skip finished right-hand sides
 };
 
+// LDS offset of neighbour nb = (dz+1)*9 + (dy+1)*3 + (dx+1) relative to the centre.
+__device__ constexpr int nboff(int nb) { return (nb / 9 - 1) * (HX * HY) + ((nb / 3) % 3 - 1) * HX + (nb % 3 - 1); }
+
 template <typename T, int NR, int MODE>
-__global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs<T> A)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stencil(StencilArgs<T> A)
 {
     __shared__ T hs[NR * 3 * HN];
     __shared__ int ps[VN];
@@ -115,13 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs<T> A)
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         if (!((live >> r) & 1u)) continue;
-        T hv[27][3];
-#pragma unroll
-        for (int nb = 0; nb < 27; nb++) {
-            const int hidx = cidx + (nb / 9 - 1) * (HX * HY) + ((nb / 3) % 3 - 1) * HX + (nb % 3 - 1);
-#pragma unroll
-            for (int c = 0; c < 3; c++) hv[nb][c] = hs[(r * 3 + c) * HN + hidx];
-        }
+        const T *hr = hs + (r * 3) * HN + cidx;  // component c of neighbour nb: hr[c * HN + nboff(nb)]
         T acc[3] = {T(0), T(0), T(0)};
         unsigned done = active ? 0u : 0xFFu;  // octants (mixed) or the whole node (homogeneous) applied
         while (true) {
@@ -136,15 +134,15 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs<T> A)
                 T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
                 for (int nb = 0; nb < 27; nb++) {
-                    t0 += S[nb * 9 + 0] * hv[nb][0];
-                    t0 += S[nb * 9 + 1] * hv[nb][1];
-                    t0 += S[nb * 9 + 2] * hv[nb][2];
-                    t1 += S[nb * 9 + 3] * hv[nb][0];
-                    t1 += S[nb * 9 + 4] * hv[nb][1];
-                    t1 += S[nb * 9 + 5] * hv[nb][2];
-                    t2 += S[nb * 9 + 6] * hv[nb][0];
-                    t2 += S[nb * 9 + 7] * hv[nb][1];
-                    t2 += S[nb * 9 + 8] * hv[nb][2];
+                    t0 += S[nb * 9 + 0] * hr[0 * HN + nboff(nb)];
+                    t0 += S[nb * 9 + 1] * hr[1 * HN + nboff(nb)];
+                    t0 += S[nb * 9 + 2] * hr[2 * HN + nboff(nb)];
+                    t1 += S[nb * 9 + 3] * hr[0 * HN + nboff(nb)];
+                    t1 += S[nb * 9 + 4] * hr[1 * HN + nboff(nb)];
+                    t1 += S[nb * 9 + 5] * hr[2 * HN + nboff(nb)];
+                    t2 += S[nb * 9 + 6] * hr[0 * HN + nboff(nb)];
+                    t2 += S[nb * 9 + 7] * hr[1 * HN + nboff(nb)];
+                    t2 += S[nb * 9 + 8] * hr[2 * HN + nboff(nb)];
                 }
                 if (hm) {
                     acc[0] += t0; acc[1] += t1; acc[2] += t2;
@@ -162,9 +160,9 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs<T> A)
                         const int nb = (PZ(b) - PZ(a) + 1) * 9 + (PY(b) - PY(a) + 1) * 3 + (PX(b) - PX(a) + 1);
 #pragma unroll
                         for (int n = 0; n < 3; n++) {
-                            t0 += D[0 * 24 + b * 3 + n] * hv[nb][n];
-                            t1 += D[1 * 24 + b * 3 + n] * hv[nb][n];
-                            t2 += D[2 * 24 + b * 3 + n] * hv[nb][n];
+                            t0 += D[0 * 24 + b * 3 + n] * hr[n * HN + nboff(nb)];
+                            t1 += D[1 * 24 + b * 3 + n] * hr[n * HN + nboff(nb)];
+                            t2 += D[2 * 24 + b * 3 + n] * hr[n * HN + nboff(nb)];
                         }
                     }
                     if (mm) {
@@ -181,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs<T> A)
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
                     A.y[((int64_t)r * 3 + j) * ns + m] = acc[j];
-                    pr += (double)hv[13][j] * (double)acc[j];
+                    pr += (double)hr[j * HN] * (double)acc[j];
                 }
                 part[r] += pr;
             } else if constexpr (MODE == kEnergy) {
@@ -189,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs<T> A)
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
                     const int64_t q = ((int64_t)r * 3 + j) * ns + m;
-                    const double uq = (double)hv[13][j], bq = A.b[q], kx = (double)acc[j];
+                    const double uq = (double)hr[j * HN], bq = A.b[q], kx = (double)acc[j];
                     e = e + 0.5 * uq * kx + bq * uq;  // f90:1336
                     const double gq = kx + bq;         // f90:1337
                     A.y[q] = (T)gq;
@@ -213,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs<T> A)
 }
 
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_energy_finish(const double *partials, int nparts, const double *cconst,
+__global__ __launch_bounds__(kFinBlock) void k_energy_finish(const double *partials, int nparts, const double *cconst,
                                                           double *results)
 {
     double tot[2 * NR];
@@ -263,15 +261,15 @@ hipError_t launch_energy(Solver &s)
     A.b = s.b;
     dispatch_stencil<double, kEnergy>(s, A);
     const int np = (int)stencil_blocks(s);
-    const dim3 one(1), blk(kBlock);
+    const dim3 one(1), fin(kFinBlock);
     const double *p = s.partials;
     switch (s.nrhs) {
-    case 1: hipLaunchKernelGGL(k_energy_finish<1>, one, blk, 0, s.stream, p, np, s.cconst, s.results); break;
-    case 2: hipLaunchKernelGGL(k_energy_finish<2>, one, blk, 0, s.stream, p, np, s.cconst, s.results); break;
-    case 3: hipLaunchKernelGGL(k_energy_finish<3>, one, blk, 0, s.stream, p, np, s.cconst, s.results); break;
-    case 4: hipLaunchKernelGGL(k_energy_finish<4>, one, blk, 0, s.stream, p, np, s.cconst, s.results); break;
-    case 5: hipLaunchKernelGGL(k_energy_finish<5>, one, blk, 0, s.stream, p, np, s.cconst, s.results); break;
-    default: hipLaunchKernelGGL(k_energy_finish<6>, one, blk, 0, s.stream, p, np, s.cconst, s.results); break;
+    case 1: hipLaunchKernelGGL(k_energy_finish<1>, one, fin, 0, s.stream, p, np, s.cconst, s.results); break;
+    case 2: hipLaunchKernelGGL(k_energy_finish<2>, one, fin, 0, s.stream, p, np, s.cconst, s.results); break;
+    case 3: hipLaunchKernelGGL(k_energy_finish<3>, one, fin, 0, s.stream, p, np, s.cconst, s.results); break;
+    case 4: hipLaunchKernelGGL(k_energy_finish<4>, one, fin, 0, s.stream, p, np, s.cconst, s.results); break;
+    case 5: hipLaunchKernelGGL(k_energy_finish<5>, one, fin, 0, s.stream, p, np, s.cconst, s.results); break;
+    default: hipLaunchKernelGGL(k_energy_finish<6>, one, fin, 0, s.stream, p, np, s.cconst, s.results); break;
     }
     return hipGetLastError();
 }
