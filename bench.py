@@ -1,0 +1,177 @@
+"""Benchmark of the elas3dxtal_pcg solve on one MI355X.
+
+Workload: the 200^3 periodic cube with 500 Voronoi grains (seeded centres, uniform
+SO(3) orientations), cubic C12 = 147.3, C44 = 124.7 GPa with Zener ratio A = 2
+(C11 = 272.0 GPa), 1% porosity as seeded lognormal spheres (median radius 3 voxels,
+sigma 0.4), macro strain exx = float32(1e-3), the reference's PCG controls
+(tol = float32(1e-8), block-Jacobi, divergence reset).
+
+One step = one complete solve on device-resident inputs: femat (tables, jump RHS),
+affine guess, energy, block-Jacobi PCG to convergence, energy, averaged stress.
+
+value = FP64 PCG DOF-updates/s = 3 * ns * Lstep / t_pcg (t_pcg: GPU time of the
+PCG loop, HIP events on the solver's stream), summed over the K timed steps.
+Also reported: solves/s and per-solve latency percentiles (host wall clock), the
+same for the mixed-precision form, the roofline of the stencil-apply kernel, and
+the CPU baseline (the oracle's C/OpenMP restatement of the reference on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "elas3d-xtal_amd"))
+
+from elas3dxtal import _lib as L  # noqa: E402
+from elas3dxtal import configs  # noqa: E402
+from elas3dxtal.solver import Elas3D  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+APPLY_BYTES_PER_NODE = {L.PREC_FP64: 6 * 8 + 4, L.PREC_MIXED: 6 * 4 + 4}  # read h, write Ah, read pix
+
+
+def solve_resident(s, c11, c12, c44, e6, precision):
+    """One step on the resident microstructure: the main program's sequence
+    (src/elas3dxtal_pcg.f90:417-519)."""
+    t0 = time.perf_counter()
+    s.femat(c11, c12, c44, e6)
+    s.affine_guess()
+    _, gg0 = s.energy()
+    d = s.dembx(gg0, precision=precision, replace_every=50, want_history=False)
+    ut, _ = s.energy()
+    st, _ = s.stress()
+    return time.perf_counter() - t0, d, ut, st
+
+
+def run_gpu(w, c11, c12, c44, e6, steps, warmup, precision):
+    s = Elas3D(w["shape"], w["nphase"])
+    try:
+        s.ppixel(w["pix"], w["rod"])
+        for _ in range(warmup):
+            solve_resident(s, c11, c12, c44, e6, precision)
+        walls, pcg_ms, lsteps, ut = [], [], [], None
+        for _ in range(steps):
+            wall, d, ut, st = solve_resident(s, c11, c12, c44, e6, precision)
+            walls.append(wall)
+            pcg_ms.append(d["pcg_ms"])
+            lsteps.append(int(d["lstep"][0]))
+            exit_flag = int(d["exit"][0])
+        apply_ms = s.time_apply(precision=precision, reps=20)
+        return dict(walls=walls, pcg_ms=pcg_ms, lsteps=lsteps, exit=exit_flag, utot=float(ut[0]),
+                    stress_xx=float(st[0][0]), apply_ms=apply_ms)
+    finally:
+        s.close()
+
+
+def cpu_baseline(w, c11, c12, c44, e6, iters):
+    """The oracle (C/OpenMP restatement of the reference) on this host: `iters` PCG
+    iterations on the same 200^3 workload, timed around oc_dembx only."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as oc
+
+    shape, pix, rod = w["shape"], w["pix"], w["rod"]
+    rot = oc.rotate_stiffness(rod, c11, c12, c44)
+    cm = oc.cmod(w["nphase"], rot, [0.0, 0.0])
+    dk = oc.dk(cm)
+    b, cval = oc.femat_rhs(shape, pix, dk, e6)
+    u0 = oc.affine(shape, e6)
+    g0, _ = oc.energy(shape, pix, dk, b, cval, u0)
+    gg0 = oc.sumsq(g0)
+    t0 = time.perf_counter()
+    r = oc.dembx(shape, pix, dk, u0, g0, gg0, ldemb=iters)
+    dt = time.perf_counter() - t0
+    ns = int(np.prod(shape))
+    return dict(value=3.0 * ns * r["lstep"] / dt, unit="DOF-updates/s", cores=int(oc.lib.oc_get_threads()),
+                kind="port",
+                sample=f"{r['lstep']} PCG iterations (block-Jacobi setup included) of the 200^3 A=2 workload "
+                       f"in {dt:.1f} s: oracle C restatement of dembx_OpenMP, OpenMP")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=200, help="grid edge (the workload is 200)")
+    ap.add_argument("--zener", type=float, default=2.0)
+    ap.add_argument("--cpu-iters", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the solve on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; independent solves shard one per GPU).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = configs.build("cfg2", n=args.n)
+    c11, c12, c44 = configs.materials("cfg2", zener=args.zener)
+    e6 = configs.exx_strain()
+    ns = int(np.prod(w["shape"]))
+
+    fp64 = run_gpu(w, c11, c12, c44, e6, args.steps, args.warmup, L.PREC_FP64)
+    t_pcg = sum(fp64["pcg_ms"]) * 1e-3
+    dofups = 3.0 * ns * sum(fp64["lsteps"]) / t_pcg
+    walls = np.array(fp64["walls"])
+
+    apply_s = fp64["apply_ms"] * 1e-3
+    alg_bytes = APPLY_BYTES_PER_NODE[L.PREC_FP64] * ns
+    achieved = alg_bytes / apply_s / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_stencil<double,1,kApply> (Ah = K h, h.Ah partials)",
+                    avg_launch_us=round(fp64["apply_ms"] * 1e3, 2),
+                    algorithmic_bytes_per_launch=alg_bytes)
+
+    out = {
+        "metric": "PCG DOF-updates/s at 200^3 (FP64)",
+        "value": dofups,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(walls.mean() * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded periodic Voronoi microstructure, generated on the host)",
+        "config": {"workload": f"{args.n}^3 periodic cube, 500 Voronoi grains, Zener A={args.zener:g}, "
+                               "1% lognormal spherical pores, exx=1e-3, reference PCG controls",
+                   "grid": list(w["shape"]), "grains": 500, "zener": args.zener,
+                   "pore_fraction": w["pore_fraction"], "dof": 3 * ns},
+        "lstep": fp64["lsteps"][0],
+        "exit": fp64["exit"],
+        "solves_per_s": float(1.0 / walls.mean()),
+        "latency_ms_p50": float(np.percentile(walls, 50) * 1e3),
+        "latency_ms_p99": float(np.percentile(walls, 99) * 1e3),
+        "pcg_ms_per_solve": float(np.mean(fp64["pcg_ms"])),
+        "utot": fp64["utot"],
+        "roofline": roofline,
+    }
+    if not args.no_mixed:
+        mx = run_gpu(w, c11, c12, c44, e6, args.steps, args.warmup, L.PREC_MIXED)
+        mwalls = np.array(mx["walls"])
+        out["mixed"] = {
+            "dof_updates_per_s": 3.0 * ns * sum(mx["lsteps"]) / (sum(mx["pcg_ms"]) * 1e-3),
+            "solves_per_s": float(1.0 / mwalls.mean()),
+            "latency_ms_p50": float(np.percentile(mwalls, 50) * 1e3),
+            "latency_ms_p99": float(np.percentile(mwalls, 99) * 1e3),
+            "lstep": mx["lsteps"][0],
+            "utot_rel_err_vs_fp64": abs(mx["utot"] - fp64["utot"]) / abs(fp64["utot"]),
+            "stress_xx_rel_err_vs_fp64": abs(mx["stress_xx"] - fp64["stress_xx"]) / abs(fp64["stress_xx"]),
+            "apply_us": round(mx["apply_ms"] * 1e3, 2),
+        }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, c11, c12, c44, e6, args.cpu_iters)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -141,9 +141,9 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     const int64_t ns = s->ns, nv = (int64_t)nrhs * 3 * ns;
     CK(dalloc(*s, &s->cmod, (int64_t)nphase * 36));
     CK(dalloc(*s, &s->dk, (int64_t)nphase * 576));
-    CK(dalloc(*s, &s->sten, (int64_t)nphase * 243));
+    CK(dalloc(*s, &s->sten, (int64_t)nphase * kStenHalf));
     CK(dalloc(*s, &s->dk32, (int64_t)nphase * 576));
-    CK(dalloc(*s, &s->sten32, (int64_t)nphase * 243));
+    CK(dalloc(*s, &s->sten32, (int64_t)nphase * kStenHalf));
     CK(dalloc(*s, &s->orient, (int64_t)3 * std::max(nphase - 1, 1)));
     CK(dalloc(*s, &s->pix, ns));
     CK(dalloc(*s, &s->u, nv));
@@ -472,6 +472,40 @@ int e3x_last_timing(e3x_solver *h, double *precond_ms, double *pcg_ms)
     Solver &s = S(h);
     if (precond_ms) *precond_ms = s.last_precond_ms;
     if (pcg_ms) *pcg_ms = s.last_pcg_ms;
+    return E3X_OK;
+}
+
+int e3x_time_apply(e3x_solver *h, int precision, int reps, double *avg_ms)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_femat) return fail(E3X_ESTATE, "e3x_time_apply before e3x_femat");
+    if (reps < 1 || !avg_ms) return fail(E3X_EINVAL, "reps must be >= 1 and avg_ms non-NULL");
+    const bool mixed = precision == E3X_PREC_MIXED;
+    E3X_HIP(hipSetDevice(s.device));
+    if (mixed)
+        if (int rc = ensure_mixed_buffers(s)) return rc;
+    // save the PCG state, mark every rhs running so the kernel does its work
+    PcgState saved[kMaxRhs];
+    E3X_HIP(hipMemcpyAsync(s.st_host, s.st, sizeof(PcgState) * s.nrhs, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    for (int r = 0; r < s.nrhs; r++) {
+        saved[r] = s.st_host[r];
+        s.st_host[r].stop_iter = 0;
+    }
+    E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
+    if (mixed) E3X_HIP(launch_to_f32(s.h, s.h32, (int64_t)s.nrhs * 3 * s.ns, s.stream));
+    E3X_HIP(launch_apply_only(s, mixed));  // warm-up
+    E3X_HIP(hipEventRecord(s.ev[0], s.stream));
+    for (int i = 0; i < reps; i++) E3X_HIP(launch_apply_only(s, mixed));
+    E3X_HIP(hipEventRecord(s.ev[1], s.stream));
+    E3X_HIP(hipEventSynchronize(s.ev[1]));
+    float ms = 0;
+    E3X_HIP(hipEventElapsedTime(&ms, s.ev[0], s.ev[1]));
+    *avg_ms = ms / reps;
+    for (int r = 0; r < s.nrhs; r++) s.st_host[r] = saved[r];
+    E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
     return E3X_OK;
 }
 

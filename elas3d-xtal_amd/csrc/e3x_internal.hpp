@@ -17,6 +17,9 @@ constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched roun
 constexpr int TX = 8, TY = 8, TZ = 4;
 constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HN = HX * HY * HZ;  // node halo
 constexpr int VX = TX + 1, VY = TY + 1, VZ = TZ + 1, VN = VX * VY * VZ;  // voxel halo
+// Homogeneous 27-point stencil per phase, symmetric half: blocks of neighbours 14..26
+// (9 each) then the centre block's upper triangle 00 11 22 01 02 12.
+constexpr int kStenHalf = 13 * 9 + 6;
 
 // Per right-hand-side PCG scalars, resident on the device (the locals of
 // dembx_OpenMP, f90:1440-1445).
@@ -42,7 +45,7 @@ struct Solver {
     double strain[6 * kMaxRhs] = {};
 
     // phase tables
-    double *cmod = nullptr, *dk = nullptr, *sten = nullptr;  // [p][36], [p][576], [p][243]
+    double *cmod = nullptr, *dk = nullptr, *sten = nullptr;  // [p][36], [p][576], [p][kStenHalf]
     float *dk32 = nullptr, *sten32 = nullptr;
     double *orient = nullptr;                                  // [3*(nphase-1)]
     uint16_t *pix = nullptr;                                   // 0-based phase per voxel
@@ -81,6 +84,7 @@ hipError_t launch_precond(Solver &s, bool want32);
 hipError_t launch_energy(Solver &s);                          // results[0..2*nrhs)
 hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K x (FP64)
 hipError_t launch_apply_hah(Solver &s, bool mixed, int it);   // Ah = K h, hAh, lambda
+hipError_t launch_apply_only(Solver &s, bool mixed);          // Ah = K h and partials
 hipError_t launch_resid(Solver &s);                           // g32 = K u + b in FP64
 // e3x_pcg.hip
 hipError_t launch_hah_finish(Solver &s, int nparts, int it);

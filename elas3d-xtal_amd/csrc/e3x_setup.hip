@@ -8,9 +8,9 @@ namespace {
 
 // ---------------------------------------------------------------------------
 // femat: one workgroup per phase (rotatestiffnessby f90:2470-2637, cmod f90:655-706,
-// dk by 3x3x3 Simpson f90:713-805) plus the homogeneous 27-point stencil blocks
-// sten[p][nb][j][n] = sum over the (voxel, local a, local b) pairs that couple a node
-// to its neighbour nb of dk[p][a][j][b][n].
+// dk by 3x3x3 Simpson f90:713-805) plus the homogeneous 27-point stencil of every
+// phase: block S[nb][j][n] = sum over the (voxel, local a, local b) pairs that couple a
+// node to its neighbour nb of dk[p][a][j][b][n], stored as its symmetric half (126).
 __device__ double shape_deriv(int a, int d, double x, double y, double z)
 {
     const double fx = PX(a) ? x : 1.0 - x, fy = PY(a) ? y : 1.0 - y, fz = PZ(a) ? z : 1.0 - z;
@@ -153,6 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_femat(int nphase, int flag_m, const 
         dk32[(int64_t)p * 576 + e] = (float)acc;
     }
     __syncthreads();
+    __shared__ double sfull[243];
     for (int e = threadIdx.x; e < 243; e += kBlock) {
         const int n = e % 3, j = (e / 3) % 3, nb = e / 9;
         const int dx = nb % 3 - 1, dy = (nb / 3) % 3 - 1, dz = nb / 9 - 1;
@@ -165,8 +166,21 @@ __global__ __launch_bounds__(kBlock) void k_femat(int nphase, int flag_m, const 
                     acc = first ? d : acc + d;
                     first = false;
                 }
-        sten[(int64_t)p * 243 + e] = acc;
-        sten32[(int64_t)p * 243 + e] = (float)acc;
+        sfull[e] = acc;
+    }
+    __syncthreads();
+    // half stencil: the 13 blocks of the neighbours nb = 14..26 (the block of -delta is
+    // the transpose, K being symmetric) and the upper triangle of the centre block
+    for (int e = threadIdx.x; e < kStenHalf; e += kBlock) {
+        double v;
+        if (e < 117) {
+            v = sfull[(14 + e / 9) * 9 + e % 9];
+        } else {
+            const int c[6] = {0, 4, 8, 1, 2, 5};  // 00 11 22 01 02 12 of the centre block
+            v = sfull[13 * 9 + c[e - 117]];
+        }
+        sten[(int64_t)p * kStenHalf + e] = v;
+        sten32[(int64_t)p * kStenHalf + e] = (float)v;
     }
 }
 

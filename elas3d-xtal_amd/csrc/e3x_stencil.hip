@@ -1,17 +1,20 @@
 // gfx950 matrix-free stencil y = K x of the elas3dxtal_pcg solve (energy_OpenMP
 // f90:1176-1321, dembx SpMV f90:1585-1884).
 //
-// A block owns an 8x8x4 tile of nodes.  It stages the 10x10x6 node halo of x and the
-// 9x9x5 voxel phases in LDS, classifies every node (homogeneous: all 8 voxels one
-// phase; mixed: otherwise), and sorts its 256 nodes by (class, phase) so that waves
-// see as few phases as possible.  Each wave then loops over the distinct phases it
-// holds; for phase P it applies, with wave-uniform coefficients read through the
-// scalar cache (SGPR operands, no per-lane coefficient loads):
-//   - to its homogeneous lanes of phase P the 27-point stencil of P (243 FMA),
+// A block owns an 8x8x4 tile of nodes and stages the 10x10x6 node halo of x and the
+// 9x9x5 voxel phases in LDS; each 64-lane wave owns a 4x4x4 sub-cube (the shape most
+// often inside one grain).  A node is homogeneous when its 8 voxels share one phase,
+// mixed otherwise.  Each wave loops over the distinct phases it holds, in ascending
+// phase order, and for phase P applies with wave-uniform coefficients read through
+// the scalar cache (SGPR operands, no per-lane coefficient loads):
+//   - to its homogeneous lanes of phase P the 27-point stencil of P (243 FMA; the
+//     block of -delta is the transpose of the block of +delta, so 126 coefficients),
 //   - to its mixed lanes, for every octant a whose voxel has phase P, the 3x24 rows of
 //     the element matrix dk[P][a] (72 FMA).
-// Neighbour values are read from LDS at their use; registers stay below 128 so four
-// or more waves per SIMD hide the scalar-cache latency of the coefficient loads.
+// A node's sum is therefore ordered by its own phases and octants only, independent of
+// the other lanes: results are bitwise reproducible.  Neighbour values are read from
+// LDS at their use, which keeps registers low enough for six waves per SIMD to hide
+// the scalar-cache latency.
 #include "e3x_device.hpp"
 
 #include <cmath>
@@ -37,12 +40,10 @@ struct StencilArgs {
 __device__ constexpr int nboff(int nb) { return (nb / 9 - 1) * (HX * HY) + ((nb / 3) % 3 - 1) * HX + (nb % 3 - 1); }
 
 template <typename T, int NR, int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stencil(StencilArgs<T> A)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_stencil(StencilArgs<T> A)
 {
     __shared__ T hs[NR * 3 * HN];
     __shared__ int ps[VN];
-    __shared__ unsigned keys[kBlock];
-    __shared__ int order[kBlock];
 
     unsigned live = (1u << NR) - 1u;  // right-hand sides to compute
     if constexpr (MODE == kApply || MODE == kResid) {
@@ -72,35 +73,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     }
     __syncthreads();
 
-    // class key of tile node t: homogeneous phase p -> p, mixed -> 0x10000, outside -> 0x10001
-    auto node_of = [&](int t, int &lx, int &ly, int &lz) {
-        lx = t % TX;
-        ly = (t / TX) % TY;
-        lz = t / (TX * TY);
-    };
-    {
-        int lx, ly, lz;
-        node_of(threadIdx.x, lx, ly, lz);
-        const bool in = ox + lx < A.nx && oy + ly < A.ny && oz + lz < A.nz;
-        const int base = lz * (VX * VY) + ly * VX + lx;
-        const int p0 = ps[base + (VX * VY) + VX + 1];
-        bool hom = true;
-#pragma unroll
-        for (int a = 1; a < 8; a++) hom &= ps[base + (1 - PZ(a)) * (VX * VY) + (1 - PY(a)) * VX + (1 - PX(a))] == p0;
-        const unsigned cls = !in ? 0x10001u : (hom ? (unsigned)p0 : 0x10000u);
-        keys[threadIdx.x] = (cls << 8) | (unsigned)threadIdx.x;
-    }
-    __syncthreads();
-    {
-        const unsigned mine = keys[threadIdx.x];
-        int rank = 0;
-        for (int i = 0; i < kBlock; i++) rank += keys[i] < mine;
-        order[rank] = threadIdx.x;
-    }
-    __syncthreads();
-
-    int lx, ly, lz;
-    node_of(order[threadIdx.x], lx, ly, lz);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int lx = (w & 1) * 4 + (l & 3), ly = (w >> 1) * 4 + ((l >> 2) & 3), lz = l >> 4;
     const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
     const bool active = gx < A.nx && gy < A.ny && gz < A.nz;
     const int64_t m = gx + (int64_t)A.nx * (gy + (int64_t)A.ny * gz);
@@ -123,27 +97,48 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         T acc[3] = {T(0), T(0), T(0)};
         unsigned done = active ? 0u : 0xFFu;  // octants (mixed) or the whole node (homogeneous) applied
         while (true) {
-            int cand = -1;
-            if (done != 0xFFu) cand = homog ? pv[0] : pv[__builtin_ctz(~done & 0xFFu)];
-            const unsigned long long has = __ballot(cand >= 0);
-            if (!has) break;
-            const int P = __builtin_amdgcn_readfirstlane(__shfl(cand, __ffsll((long long)has) - 1, 64));
+            // smallest phase this lane still needs; the wave takes the smallest of all
+            int cand = 0x7fffffff;
+            if (done != 0xFFu) {
+                if (homog) {
+                    cand = pv[0];
+                } else {
+#pragma unroll
+                    for (int a = 0; a < 8; a++)
+                        if (!((done >> a) & 1u)) cand = min(cand, pv[a]);
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) cand = min(cand, __shfl_xor(cand, off, 64));
+            if (cand == 0x7fffffff) break;
+            const int P = __builtin_amdgcn_readfirstlane(cand);
             const bool hm = homog && done != 0xFFu && pv[0] == P;
             if (__ballot(hm)) {
-                const T *S = A.sten + (int64_t)P * 243;
+                const T *S = A.sten + (int64_t)P * kStenHalf;
                 T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
-                for (int nb = 0; nb < 27; nb++) {
-                    t0 += S[nb * 9 + 0] * hr[0 * HN + nboff(nb)];
-                    t0 += S[nb * 9 + 1] * hr[1 * HN + nboff(nb)];
-                    t0 += S[nb * 9 + 2] * hr[2 * HN + nboff(nb)];
-                    t1 += S[nb * 9 + 3] * hr[0 * HN + nboff(nb)];
-                    t1 += S[nb * 9 + 4] * hr[1 * HN + nboff(nb)];
-                    t1 += S[nb * 9 + 5] * hr[2 * HN + nboff(nb)];
-                    t2 += S[nb * 9 + 6] * hr[0 * HN + nboff(nb)];
-                    t2 += S[nb * 9 + 7] * hr[1 * HN + nboff(nb)];
-                    t2 += S[nb * 9 + 8] * hr[2 * HN + nboff(nb)];
+                for (int k = 0; k < 13; k++) {
+                    const int np = 14 + k, nm = 12 - k;  // neighbours +delta and -delta
+#pragma unroll
+                    for (int n = 0; n < 3; n++) {
+                        const T hp = hr[n * HN + nboff(np)];
+                        t0 += S[k * 9 + 0 * 3 + n] * hp;
+                        t1 += S[k * 9 + 1 * 3 + n] * hp;
+                        t2 += S[k * 9 + 2 * 3 + n] * hp;
+                    }
+#pragma unroll
+                    for (int n = 0; n < 3; n++) {  // block of -delta: S[-d][j][n] = S[d][n][j]
+                        const T hq = hr[n * HN + nboff(nm)];
+                        t0 += S[k * 9 + n * 3 + 0] * hq;
+                        t1 += S[k * 9 + n * 3 + 1] * hq;
+                        t2 += S[k * 9 + n * 3 + 2] * hq;
+                    }
                 }
+                const T *Cc = S + 117;  // centre: 00 11 22 01 02 12
+                const T c0 = hr[0], c1 = hr[HN], c2 = hr[2 * HN];
+                t0 += Cc[0] * c0; t0 += Cc[3] * c1; t0 += Cc[4] * c2;
+                t1 += Cc[3] * c0; t1 += Cc[1] * c1; t1 += Cc[5] * c2;
+                t2 += Cc[4] * c0; t2 += Cc[5] * c1; t2 += Cc[2] * c2;
                 if (hm) {
                     acc[0] += t0; acc[1] += t1; acc[2] += t2;
                     done = 0xFFu;
@@ -286,6 +281,13 @@ hipError_t launch_apply_hah(Solver &s, bool mixed, int it)
     if (mixed) dispatch_stencil<float, kApply>(s, stencil_args<float>(s, s.dk32, s.sten32, s.h32, s.ah32));
     else dispatch_stencil<double, kApply>(s, stencil_args<double>(s, s.dk, s.sten, s.h, s.ah));
     return launch_hah_finish(s, (int)stencil_blocks(s), it);
+}
+
+hipError_t launch_apply_only(Solver &s, bool mixed)
+{
+    if (mixed) dispatch_stencil<float, kApply>(s, stencil_args<float>(s, s.dk32, s.sten32, s.h32, s.ah32));
+    else dispatch_stencil<double, kApply>(s, stencil_args<double>(s, s.dk, s.sten, s.h, s.ah));
+    return hipGetLastError();
 }
 
 hipError_t launch_resid(Solver &s)

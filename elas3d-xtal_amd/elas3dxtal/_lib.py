@@ -52,6 +52,7 @@ SIGNATURES = {
     "e3x_set_field": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "e3x_get_tables": (C.c_int, [_vp, _dp, _dp]),
     "e3x_last_timing": (C.c_int, [_vp, _dp, _dp]),
+    "e3x_time_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
 }
 
 _lib = None

@@ -152,6 +152,13 @@ class Elas3D:
             raise L.E3xError(L.E3X_EINVAL, "field size must be 3*ns")
         L.check(self._lib.e3x_set_field(self._h, which, rhs, L.dptr(v)))
 
+    def time_apply(self, precision=L.PREC_FP64, reps=20):
+        """Average GPU time (ms) of the PCG's stencil-apply kernel, HIP events on the
+        solver's stream (e3x_time_apply)."""
+        ms = C.c_double()
+        L.check(self._lib.e3x_time_apply(self._h, precision, reps, C.byref(ms)))
+        return ms.value
+
     def tables(self):
         cm = np.zeros(36 * self.nphase)
         dk = np.zeros(576 * self.nphase)

@@ -169,6 +169,15 @@ int e3x_get_tables(e3x_solver *s, double *cmod, double *dk);
 /* GPU time of the last e3x_dembx(_ex) call: preconditioner build and the PCG loop (ms). */
 int e3x_last_timing(e3x_solver *s, double *precond_ms, double *pcg_ms);
 
+/*
+ * Measurement hook (no counterpart in the reference): replay `reps` launches of the
+ * PCG's stencil-apply kernel (Ah = K h with the h.Ah block partials, FP64 or the mixed
+ * form's FP32) on the solver's stream, bracketed by HIP events on that stream, and
+ * return the average launch time in ms.  The PCG state is saved and restored; h, Ah
+ * and the partials are overwritten.  Needs e3x_femat first.
+ */
+int e3x_time_apply(e3x_solver *s, int precision, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
