@@ -90,7 +90,7 @@ int e3x_memory_bytes(const e3x_solver *s, int64_t *bytes);
  * Load the microstructure: pix[ns] with labels in [1, nphase] (checked,
  * E3X_EPHASE otherwise) and orientation[3*(nphase-1)] Rodrigues vectors
  * tan(theta/2)*axis, grain-major.  Data part of ppixel_hdf5 (f90:2879-2991);
- * the HDF5 file reading itself is in e3x_h5.h.  orientation may be NULL for
+ * the HDF5 file reading itself is in e3x_io.h.  orientation may be NULL for
  * flag_m = 0.
  */
 int e3x_set_microstructure(e3x_solver *s, const int32_t *pix, const double *orientation);
