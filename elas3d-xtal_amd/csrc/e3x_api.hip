@@ -156,7 +156,7 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->cconst, kMaxRhs));
     // partials: [values][blocks] of the largest reduction (stress: 12 per rhs over the
     // streaming grid; energy: 2 per rhs over the stencil grid; rhs constant: 6)
-    s->partials_cap = std::max<int64_t>(12 * (int64_t)nrhs * vector_grid(*s), 2 * (int64_t)nrhs * stencil_blocks(*s));
+    s->partials_cap = std::max<int64_t>(12 * (int64_t)nrhs * vector_grid(*s), 2 * (int64_t)nrhs * stencil_parts(*s));
     s->partials_cap = std::max<int64_t>(s->partials_cap, (int64_t)kMaxRhs * vector_grid(*s));
     CK(dalloc(*s, &s->partials, s->partials_cap));
     CK(dalloc(*s, &s->st, kMaxRhs));

@@ -23,10 +23,26 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return xcd * q + (xcd < rem ? xcd : rem) + k;
 }
 
+// v of lane i - SHIFT within its row of 16 lanes (row_shr), 0 where that runs off the
+// row; CTRL >= 0x142 are the row broadcasts of lane 15 / 31 into the rows in ROWS.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave (all lanes active), valid in lane 63: a fixed DPP
+// tree (row scans, then row broadcasts), so the order of the additions never varies.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    v += dpp_f64<0x111>(v);        // row_shr:1
+    v += dpp_f64<0x112>(v);        // row_shr:2
+    v += dpp_f64<0x114>(v);        // row_shr:4
+    v += dpp_f64<0x118>(v);        // row_shr:8  -> lane 15 of each row holds the row sum
+    v += dpp_f64<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
+    v += dpp_f64<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3 -> lane 63 holds the total
     return v;
 }
 
@@ -41,7 +57,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         const double t = wave_sum(v[q]);
-        if (l == 0) sh[q][w] = t;
+        if (l == 63) sh[q][w] = t;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -63,6 +79,21 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *partials
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; q++) partials[(int64_t)q * gridDim.x + blockIdx.x] = v[q];
+    }
+}
+
+// Per-wave partials, layout [q][wave of the grid]: no block barrier, so a wave that is
+// done leaves at once instead of waiting for the slowest wave of its block.  Called by
+// every lane of the wave (all lanes active).
+template <int NV>
+__device__ __forceinline__ void write_wave_partials(double (&v)[NV], double *partials)
+{
+    const int nw = gridDim.x * (kBlock / 64);
+    const int wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        const double t = wave_sum(v[q]);
+        if ((threadIdx.x & 63) == 63) partials[(int64_t)q * nw + wid] = t;
     }
 }
 

@@ -2,19 +2,22 @@
 // f90:1176-1321, dembx SpMV f90:1585-1884).
 //
 // A block owns an 8x8x4 tile of nodes and stages the 10x10x6 node halo of x and the
-// 9x9x5 voxel phases in LDS; each 64-lane wave owns a 4x4x4 sub-cube (the shape most
-// often inside one grain).  A node is homogeneous when its 8 voxels share one phase,
-// mixed otherwise.  Each wave loops over the distinct phases it holds, in ascending
-// phase order, and for phase P applies with wave-uniform coefficients read through
-// the scalar cache (SGPR operands, no per-lane coefficient loads):
-//   - to its homogeneous lanes of phase P the 27-point stencil of P (243 FMA; the
-//     block of -delta is the transpose of the block of +delta, so 126 coefficients),
-//   - to its mixed lanes, for every octant a whose voxel has phase P, the 3x24 rows of
-//     the element matrix dk[P][a] (72 FMA).
-// A node's sum is therefore ordered by its own phases and octants only, independent of
-// the other lanes: results are bitwise reproducible.  Neighbour values are read from
-// LDS at their use, which keeps registers low enough for six waves per SIMD to hide
-// the scalar-cache latency.
+// 9x9x5 voxel phases in LDS.  A node is homogeneous when its 8 voxels share one phase,
+// mixed otherwise.  After staging, the block packs its homogeneous nodes into the
+// leading lanes and its mixed nodes into the trailing ones (a stable ballot
+// compaction), so that most waves hold one kind only.  Coefficients are wave-uniform
+// and read through the scalar cache (SGPR operands, no per-lane coefficient loads):
+//   - homogeneous lanes: the wave loops over the phases its lanes hold; for phase P the
+//     27-point stencil of P (243 FMA; the block of -delta is the transpose of the block
+//     of +delta, so 126 coefficients) goes to the lanes of phase P;
+//   - mixed lanes: for each octant a in order 0..7 the wave reads the 24 neighbour
+//     values of voxel a once, then loops over the phases its lanes have at octant a and
+//     applies the 3x24 rows of the element matrix dk[P][a] (72 FMA) to those lanes.
+// The next phase is the one of the first lane still pending (ballot + readlane).  Each
+// lane is served exactly once per octant (or once in all, when homogeneous), so a
+// node's sum runs over its octants in fixed order whatever its neighbours hold: results
+// are bitwise reproducible.  Right-hand sides are staged one at a time, which keeps the
+// LDS of a block at 17 KB (FP64) for any number of them.
 #include "e3x_device.hpp"
 
 #include <cmath>
@@ -36,55 +39,110 @@ struct StencilArgs {
     const PcgState *st;    // kApply, kResid: skip finished right-hand sides
 };
 
-// LDS offset of neighbour nb = (dz+1)*9 + (dy+1)*3 + (dx+1) relative to the centre.
-__device__ constexpr int nboff(int nb) { return (nb / 9 - 1) * (HX * HY) + ((nb / 3) % 3 - 1) * HX + (nb % 3 - 1); }
+__device__ __forceinline__ int pmod(int v, int n) { v %= n; return v < 0 ? v + n : v; }
 
+// Component c of neighbour nb = (dz+1)*9 + (dy+1)*3 + (dx+1) relative to the LDS centre hr.
+#define E3X_NB(hr, c, nb) (hr)[(c) * HN + ((nb) / 9 - 1) * (HX * HY) + ((nb) / 3 % 3 - 1) * HX + ((nb) % 3 - 1)]
+
+// The body takes the arrays as restrict parameters: the output stores of one right-hand
+// side then provably leave the tables untouched, and the coefficient loads of the next
+// one stay scalar.
 template <typename T, int NR, int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_stencil(StencilArgs<T> A)
+__device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint16_t *__restrict__ pix,
+                                             const T *__restrict__ dk, const T *__restrict__ sten,
+                                             const T *__restrict__ x, T *__restrict__ y,
+                                             const double *__restrict__ bvec, float *__restrict__ y32)
 {
-    __shared__ T hs[NR * 3 * HN];
+    __shared__ T hs[3 * HN];
     __shared__ int ps[VN];
+    __shared__ int cnt[kBlock / 64];
+    __shared__ short order[kBlock];
 
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = bid % A.ntx, ty = (bid / A.ntx) % A.nty, tz = bid / (A.ntx * A.nty);
+    const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
+    const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
+    const int64_t ns = A.ns;
+    const int tid = threadIdx.x;
+    // Interior tiles index without wrapping; boundary tiles wrap each coordinate.
+    const bool interior = ox >= 1 && oy >= 1 && oz >= 1 && ox + TX + 1 <= nx && oy + TY + 1 <= ny && oz + TZ + 1 <= nz;
+    const int corner = (oz - 1) * nxy + (oy - 1) * nx + (ox - 1);
+    auto stage = [&](int r) {
+        const T *x0 = x + (int64_t)(3 * r) * ns, *x1 = x0 + ns, *x2 = x1 + ns;
+        if (interior) {  // unrolled: all loads of a thread in flight together
+#pragma unroll
+            for (int it = 0; it < (HN + kBlock - 1) / kBlock; it++) {
+                const int q = tid + it * kBlock;
+                if (q < HN) {
+                    const int g = corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX;
+                    hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+                }
+            }
+        } else {
+            for (int q = tid; q < HN; q += kBlock) {
+                const int g = pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
+                              nxy * pmod(oz + q / (HX * HY) - 1, nz);
+                hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+            }
+        }
+    };
+
+    stage(0);  // issued before the state check below, so its latency overlaps it
+    if (interior) {
+#pragma unroll
+        for (int it = 0; it < (VN + kBlock - 1) / kBlock; it++) {
+            const int q = tid + it * kBlock;
+            if (q < VN) ps[q] = pix[corner + (q / (VX * VY)) * nxy + ((q / VX) % VY) * nx + q % VX];
+        }
+    } else {
+        for (int q = tid; q < VN; q += kBlock)
+            ps[q] = pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
+                        nxy * pmod(oz + q / (VX * VY) - 1, nz)];
+    }
     unsigned live = (1u << NR) - 1u;  // right-hand sides to compute
     if constexpr (MODE == kApply || MODE == kResid) {
         live = 0u;
 #pragma unroll
         for (int r = 0; r < NR; r++)
             if (A.st[r].stop_iter == 0) live |= 1u << r;
-        if (!live) return;
-    }
-
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = bid % A.ntx, ty = (bid / A.ntx) % A.nty, tz = bid / (A.ntx * A.nty);
-    const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
-    const int64_t ns = A.ns;
-
-    for (int q = threadIdx.x; q < HN; q += kBlock) {
-        const int hx = q % HX, hy = (q / HX) % HY, hz = q / (HX * HY);
-        const int gx = wrap(ox + hx - 1, A.nx), gy = wrap(oy + hy - 1, A.ny), gz = wrap(oz + hz - 1, A.nz);
-        const int64_t gm = gx + (int64_t)A.nx * (gy + (int64_t)A.ny * gz);
-#pragma unroll
-        for (int rc = 0; rc < NR * 3; rc++) hs[rc * HN + q] = A.x[(int64_t)rc * ns + gm];
-    }
-    for (int q = threadIdx.x; q < VN; q += kBlock) {
-        const int vx = q % VX, vy = (q / VX) % VY, vz = q / (VX * VY);
-        const int gx = wrap(ox + vx - 1, A.nx), gy = wrap(oy + vy - 1, A.ny), gz = wrap(oz + vz - 1, A.nz);
-        ps[q] = A.pix[gx + (int64_t)A.nx * (gy + (int64_t)A.ny * gz)];
+        if (!live) return;  // uniform over the block
     }
     __syncthreads();
 
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int lx = (w & 1) * 4 + (l & 3), ly = (w >> 1) * 4 + ((l >> 2) & 3), lz = l >> 4;
+    // Stable compaction: homogeneous nodes first, then mixed ones.
+    const int w = tid >> 6, l = tid & 63;
+    {
+        const int lx = (w & 1) * 4 + (l & 3), ly = (w >> 1) * 4 + ((l >> 2) & 3), lz = l >> 4;
+        const int *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
+        const int p0 = pc[0];
+        bool hom = true;
+#pragma unroll
+        for (int a = 1; a < 8; a++) hom &= pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))] == p0;
+        const unsigned long long bh = __ballot(hom);
+        if (l == 0) cnt[w] = __popcll(bh);
+        __syncthreads();
+        int nh = 0, before = 0;
+#pragma unroll
+        for (int q = 0; q < kBlock / 64; q++) { nh += cnt[q]; before += q < w ? cnt[q] : 0; }
+        const int rh = __popcll(bh & ((1ull << l) - 1ull));
+        order[hom ? before + rh : nh + (w * 64 - before) + (l - rh)] = (short)tid;
+        __syncthreads();
+    }
+    const int t = order[tid];
+    const int tw = t >> 6, tl = t & 63;
+    const int lx = (tw & 1) * 4 + (tl & 3), ly = (tw >> 1) * 4 + ((tl >> 2) & 3), lz = tl >> 4;
     const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
-    const bool active = gx < A.nx && gy < A.ny && gz < A.nz;
-    const int64_t m = gx + (int64_t)A.nx * (gy + (int64_t)A.ny * gz);
+    const bool active = gx < nx && gy < ny && gz < nz;
+    const int m = gx + nx * gy + nxy * gz;
+    const int *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
     int pv[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) pv[a] = ps[(lz + 1 - PZ(a)) * (VX * VY) + (ly + 1 - PY(a)) * VX + (lx + 1 - PX(a))];
+    for (int a = 0; a < 8; a++) pv[a] = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
     bool homog = true;
 #pragma unroll
     for (int a = 1; a < 8; a++) homog &= pv[a] == pv[0];
-    const int cidx = (lz + 1) * (HX * HY) + (ly + 1) * HX + (lx + 1);
+    const bool hlane = active && homog, mlane = active && !homog;
+    const T *hr = hs + (lz + 1) * (HX * HY) + (ly + 1) * HX + (lx + 1);
 
     double part[MODE == kEnergy ? 2 * NR : NR];
 #pragma unroll
@@ -93,42 +151,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         if (!((live >> r) & 1u)) continue;
-        const T *hr = hs + (r * 3) * HN + cidx;  // component c of neighbour nb: hr[c * HN + nboff(nb)]
-        T acc[3] = {T(0), T(0), T(0)};
-        unsigned done = active ? 0u : 0xFFu;  // octants (mixed) or the whole node (homogeneous) applied
-        while (true) {
-            // smallest phase this lane still needs; the wave takes the smallest of all
-            int cand = 0x7fffffff;
-            if (done != 0xFFu) {
-                if (homog) {
-                    cand = pv[0];
-                } else {
-#pragma unroll
-                    for (int a = 0; a < 8; a++)
-                        if (!((done >> a) & 1u)) cand = min(cand, pv[a]);
-                }
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) cand = min(cand, __shfl_xor(cand, off, 64));
-            if (cand == 0x7fffffff) break;
-            const int P = __builtin_amdgcn_readfirstlane(cand);
-            const bool hm = homog && done != 0xFFu && pv[0] == P;
-            if (__ballot(hm)) {
-                const T *S = A.sten + (int64_t)P * kStenHalf;
+        if (r != 0) {
+            __syncthreads();  // every wave is done with the previous right-hand side
+            stage(r);
+            __syncthreads();
+        }
+        T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+        if (__ballot(hlane)) {
+            bool pend = hlane;
+            while (true) {
+                const unsigned long long pm = __ballot(pend);
+                if (pm == 0ull) break;
+                const int P = __builtin_amdgcn_readlane(pv[0], __builtin_ctzll(pm));
+                const T *S = sten + (int64_t)P * kStenHalf;
                 T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
                 for (int k = 0; k < 13; k++) {
-                    const int np = 14 + k, nm = 12 - k;  // neighbours +delta and -delta
 #pragma unroll
-                    for (int n = 0; n < 3; n++) {
-                        const T hp = hr[n * HN + nboff(np)];
+                    for (int n = 0; n < 3; n++) {  // neighbour +delta
+                        const T hp = E3X_NB(hr, n, 14 + k);
                         t0 += S[k * 9 + 0 * 3 + n] * hp;
                         t1 += S[k * 9 + 1 * 3 + n] * hp;
                         t2 += S[k * 9 + 2 * 3 + n] * hp;
                     }
 #pragma unroll
-                    for (int n = 0; n < 3; n++) {  // block of -delta: S[-d][j][n] = S[d][n][j]
-                        const T hq = hr[n * HN + nboff(nm)];
+                    for (int n = 0; n < 3; n++) {  // neighbour -delta: S[-d][j][n] = S[d][n][j]
+                        const T hq = E3X_NB(hr, n, 12 - k);
                         t0 += S[k * 9 + n * 3 + 0] * hq;
                         t1 += S[k * 9 + n * 3 + 1] * hq;
                         t2 += S[k * 9 + n * 3 + 2] * hq;
@@ -139,41 +187,51 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
                 t0 += Cc[0] * c0; t0 += Cc[3] * c1; t0 += Cc[4] * c2;
                 t1 += Cc[3] * c0; t1 += Cc[1] * c1; t1 += Cc[5] * c2;
                 t2 += Cc[4] * c0; t2 += Cc[5] * c1; t2 += Cc[2] * c2;
-                if (hm) {
-                    acc[0] += t0; acc[1] += t1; acc[2] += t2;
-                    done = 0xFFu;
+                if (pend && pv[0] == P) {
+                    acc0 = t0; acc1 = t1; acc2 = t2;
+                    pend = false;
                 }
             }
-#pragma unroll
+        }
+        if (__ballot(mlane)) {
+#pragma unroll 1
             for (int a = 0; a < 8; a++) {
-                const bool mm = !homog && !((done >> a) & 1u) && pv[a] == P;
-                if (__ballot(mm)) {
-                    const T *D = A.dk + (int64_t)P * 576 + a * 72;
+                // voxel a of the node spans nodes (PX(b)-PX(a), PY(b)-PY(a), PZ(b)-PZ(a)), b = 0..7
+                const T *hb = hr - (PZ(a) * (HX * HY) + PY(a) * HX + PX(a));
+                T hv[24];
+#pragma unroll
+                for (int b = 0; b < 8; b++)
+#pragma unroll
+                    for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
+                const int pa = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
+                bool pend = mlane;
+                while (true) {
+                    const unsigned long long pm = __ballot(pend);
+                    if (pm == 0ull) break;
+                    const int P = __builtin_amdgcn_readlane(pa, __builtin_ctzll(pm));
+                    const T *D = dk + (int64_t)P * 576 + a * 72;
                     T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
-                    for (int b = 0; b < 8; b++) {
-                        const int nb = (PZ(b) - PZ(a) + 1) * 9 + (PY(b) - PY(a) + 1) * 3 + (PX(b) - PX(a) + 1);
-#pragma unroll
-                        for (int n = 0; n < 3; n++) {
-                            t0 += D[0 * 24 + b * 3 + n] * hr[n * HN + nboff(nb)];
-                            t1 += D[1 * 24 + b * 3 + n] * hr[n * HN + nboff(nb)];
-                            t2 += D[2 * 24 + b * 3 + n] * hr[n * HN + nboff(nb)];
-                        }
+                    for (int j = 0; j < 24; j++) {
+                        t0 += D[j] * hv[j];
+                        t1 += D[24 + j] * hv[j];
+                        t2 += D[48 + j] * hv[j];
                     }
-                    if (mm) {
-                        acc[0] += t0; acc[1] += t1; acc[2] += t2;
-                        done |= 1u << a;
+                    if (pend && pa == P) {
+                        acc0 += t0; acc1 += t1; acc2 += t2;
+                        pend = false;
                     }
                 }
             }
         }
 
         if (active) {
+            const T acc[3] = {acc0, acc1, acc2};
             if constexpr (MODE == kApply) {
                 double pr = 0.0;
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    A.y[((int64_t)r * 3 + j) * ns + m] = acc[j];
+                    y[(int64_t)(3 * r + j) * ns + m] = acc[j];
                     pr += (double)hr[j * HN] * (double)acc[j];
                 }
                 part[r] += pr;
@@ -181,11 +239,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
                 double e = 0.0, gsq = 0.0;
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    const int64_t q = ((int64_t)r * 3 + j) * ns + m;
-                    const double uq = (double)hr[j * HN], bq = A.b[q], kx = (double)acc[j];
+                    const int64_t q = (int64_t)(3 * r + j) * ns + m;
+                    const double uq = (double)hr[j * HN], bq = bvec[q], kx = (double)acc[j];
                     e = e + 0.5 * uq * kx + bq * uq;  // f90:1336
                     const double gq = kx + bq;         // f90:1337
-                    A.y[q] = (T)gq;
+                    y[q] = (T)gq;
                     gsq += gq * gq;                     // f90:452
                 }
                 part[r] += e;
@@ -193,16 +251,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
             } else if constexpr (MODE == kResid) {
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    const int64_t q = ((int64_t)r * 3 + j) * ns + m;
-                    A.y32[q] = (float)((double)acc[j] + A.b[q]);
+                    const int64_t q = (int64_t)(3 * r + j) * ns + m;
+                    y32[q] = (float)((double)acc[j] + bvec[q]);
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < 3; j++) A.y[((int64_t)r * 3 + j) * ns + m] = acc[j];
+                for (int j = 0; j < 3; j++) y[(int64_t)(3 * r + j) * ns + m] = acc[j];
             }
         }
     }
-    if constexpr (MODE == kApply || MODE == kEnergy) write_partials(part, A.partials);
+    if constexpr (MODE == kApply || MODE == kEnergy) write_wave_partials(part, A.partials);
+}
+#undef E3X_NB
+
+// Six waves per SIMD: enough to cover the scalar-cache latency of the coefficients,
+// within the 80 VGPRs the body needs.
+template <typename T, int NR, int MODE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_stencil(StencilArgs<T> A)
+{
+    stencil_body<T, NR, MODE>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32);
 }
 
 template <int NR>
@@ -250,12 +317,14 @@ int64_t stencil_blocks(const Solver &s)
     return (int64_t)((s.nx + TX - 1) / TX) * ((s.ny + TY - 1) / TY) * ((s.nz + TZ - 1) / TZ);
 }
 
+int64_t stencil_parts(const Solver &s) { return stencil_blocks(s) * (kBlock / 64); }
+
 hipError_t launch_energy(Solver &s)
 {
     StencilArgs<double> A = stencil_args<double>(s, s.dk, s.sten, s.u, s.g);
     A.b = s.b;
     dispatch_stencil<double, kEnergy>(s, A);
-    const int np = (int)stencil_blocks(s);
+    const int np = (int)stencil_parts(s);
     const dim3 one(1), fin(kFinBlock);
     const double *p = s.partials;
     switch (s.nrhs) {
@@ -280,7 +349,7 @@ hipError_t launch_apply_hah(Solver &s, bool mixed, int it)
 {
     if (mixed) dispatch_stencil<float, kApply>(s, stencil_args<float>(s, s.dk32, s.sten32, s.h32, s.ah32));
     else dispatch_stencil<double, kApply>(s, stencil_args<double>(s, s.dk, s.sten, s.h, s.ah));
-    return launch_hah_finish(s, (int)stencil_blocks(s), it);
+    return launch_hah_finish(s, (int)stencil_parts(s), it);
 }
 
 hipError_t launch_apply_only(Solver &s, bool mixed)
