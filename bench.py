@@ -31,7 +31,10 @@ from elas3dxtal import configs  # noqa: E402
 from elas3dxtal.solver import Elas3D  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-APPLY_BYTES_PER_NODE = {L.PREC_FP64: 6 * 8 + 4, L.PREC_MIXED: 6 * 4 + 4}  # read h, write Ah, read pix
+# Algorithmic bytes per node of the PCG apply kernel (k_stencil, fused form used from the
+# second iteration on): read z, h_prev, u and the uint16 phase id; write h, Ah and u.
+# z, h, Ah are FP64 (FP32 in the mixed form); u is FP64 in both.
+APPLY_BYTES_PER_NODE = {L.PREC_FP64: 6 * 3 * 8 + 2, L.PREC_MIXED: 4 * 3 * 4 + 2 * 3 * 8 + 2}
 
 
 def solve_resident(s, c11, c12, c44, e6, precision):
@@ -123,7 +126,8 @@ def main():
     achieved = alg_bytes / apply_s / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_stencil<double,1,kApply> (Ah = K h, h.Ah partials)",
+                    kernel="k_stencil<double,1,kApplyFused> (h = z + gamma h_prev, u -= lambda h_prev, "
+                           "Ah = K h, h.Ah partials)",
                     avg_launch_us=round(fp64["apply_ms"] * 1e3, 2),
                     algorithmic_bytes_per_launch=alg_bytes)
 

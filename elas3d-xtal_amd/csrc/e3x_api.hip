@@ -43,8 +43,8 @@ hipError_t dalloc(Solver &s, T **p, int64_t n)
 void release(Solver &s)
 {
     (void)hipSetDevice(s.device);
-    void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.u, s.g, s.b, s.h, s.z,
-                   s.ah, s.minv, s.g32, s.h32, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
+    void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.u, s.g, s.b, s.h, s.h_alt, s.z,
+                   s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
                    s.st, s.hist, s.results, s.post};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -70,6 +70,7 @@ int ensure_mixed_buffers(Solver &s)
     const int64_t nv = (int64_t)s.nrhs * 3 * s.ns;
     E3X_HIP(dalloc(s, &s.g32, nv));
     E3X_HIP(dalloc(s, &s.h32, nv));
+    E3X_HIP(dalloc(s, &s.h32_alt, nv));
     E3X_HIP(dalloc(s, &s.z32, nv));
     E3X_HIP(dalloc(s, &s.ah32, nv));
     E3X_HIP(dalloc(s, &s.minv32, 6 * s.ns));
@@ -150,6 +151,7 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->g, nv));
     CK(dalloc(*s, &s->b, nv));
     CK(dalloc(*s, &s->h, nv));
+    CK(dalloc(*s, &s->h_alt, nv));
     CK(dalloc(*s, &s->z, nv));
     CK(dalloc(*s, &s->ah, nv));
     CK(dalloc(*s, &s->minv, 6 * ns));
@@ -337,7 +339,8 @@ int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *
         for (int t = 0; t < n; t++) {
             const int itt = it + t;
             const bool replace = mixed && replace_every > 0 && itt % replace_every == 0;
-            E3X_HIP(launch_pcg_iter(s, mixed, itt, replace));
+            const bool prev_replace = mixed && replace_every > 0 && itt > 1 && (itt - 1) % replace_every == 0;
+            E3X_HIP(launch_pcg_iter(s, mixed, itt, replace, prev_replace));
         }
         it += n;
         // copy the stop_iter of every rhs into the pinned slot of this chunk
@@ -352,6 +355,8 @@ int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *
         }
         k++;
     }
+    // the last iteration's u step and direction update (deferred into the next apply)
+    E3X_HIP(launch_pcg_finalize(s, mixed, mixed ? replace_every : 0));
     E3X_HIP(hipEventRecord(s.ev[2], s.stream));
     if (mixed) {
         E3X_HIP(launch_to_f64(s.g32, s.g, nv, s.stream));
@@ -492,6 +497,9 @@ int e3x_time_apply(e3x_solver *h, int precision, int reps, double *avg_ms)
     for (int r = 0; r < s.nrhs; r++) {
         saved[r] = s.st_host[r];
         s.st_host[r].stop_iter = 0;
+        s.st_host[r].lambda = 0.0;  // u stays as it is
+        s.st_host[r].gamma = 0.0;   // h = z
+        s.st_host[r].reset = 0;
     }
     E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
     if (mixed) E3X_HIP(launch_to_f32(s.h, s.h32, (int64_t)s.nrhs * 3 * s.ns, s.stream));

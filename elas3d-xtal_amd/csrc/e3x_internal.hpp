@@ -34,7 +34,10 @@ struct PcgState {
     int pad[2];
 };
 
-enum StencilMode { kApply = 0, kEnergy = 1, kResid = 2, kPlain = 3 };
+// kApplyFused: the PCG apply of iterations >= 2, which first forms this iteration's
+// direction h = z + gamma h_prev (h = z on a reset) and takes the previous step into u
+// (u -= lambda h_prev), then applies K to the new h.
+enum StencilMode { kApply = 0, kEnergy = 1, kResid = 2, kPlain = 3, kApplyFused = 4 };
 
 struct Solver {
     int nx = 0, ny = 0, nz = 0, nphase = 0, nrhs = 1, device = 0;
@@ -54,6 +57,11 @@ struct Solver {
     double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;
     double *minv = nullptr;                                    // [6][ns]
     float *g32 = nullptr, *h32 = nullptr, *z32 = nullptr, *ah32 = nullptr, *minv32 = nullptr;
+    // Second direction buffer: the direction of PCG iteration k lives in h (k odd) or
+    // h_alt (k even), since the apply that forms it reads the previous one as halo.
+    // h holds the final direction after e3x_dembx.
+    double *h_alt = nullptr;
+    float *h32_alt = nullptr;
     double *cconst = nullptr;                                  // C per rhs
     double *post = nullptr;                                    // full-field staging [7][ns]
 
@@ -83,14 +91,18 @@ hipError_t launch_precond(Solver &s, bool want32);
 // e3x_stencil.hip
 hipError_t launch_energy(Solver &s);                          // results[0..2*nrhs)
 hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K x (FP64)
-hipError_t launch_apply_hah(Solver &s, bool mixed, int it);   // Ah = K h, hAh, lambda
-hipError_t launch_apply_only(Solver &s, bool mixed);          // Ah = K h and partials
+// PCG apply of iteration it (fused direction / u step for it >= 2; fuse_u false when
+// the previous iteration already moved u), then hAh and lambda
+hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u);
+hipError_t launch_apply_only(Solver &s, bool mixed);          // fused apply with the current state
 hipError_t launch_resid(Solver &s);                           // g32 = K u + b in FP64
 // e3x_pcg.hip
 hipError_t launch_hah_finish(Solver &s, int nparts, int it);
 hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
                            const double *gg0_host);
-hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace);
+hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace, bool prev_replace);
+// after the loop: the step still owed to u and the final direction, into h (h32)
+hipError_t launch_pcg_finalize(Solver &s, bool mixed, int replace_every);
 // e3x_post.hip
 hipError_t launch_stress(Solver &s);                          // results[0..12*nrhs)
 hipError_t launch_fullfield(Solver &s, int rhs, double *stress_dev, double *vm_dev);

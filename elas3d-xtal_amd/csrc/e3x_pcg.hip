@@ -1,13 +1,16 @@
 // gfx950 PCG loop of dembx_OpenMP (src/elas3dxtal_pcg.f90:1356-2043).
 //
-// One iteration = five launches on the solver's stream, all gated on device-resident
+// One iteration = four launches on the solver's stream, all gated on device-resident
 // per-rhs stop state (so the host can enqueue iterations ahead and stop launching
 // when it sees every right-hand side finished):
-//   stencil  Ah = A h, block partials of h.Ah        (e3x_stencil.hip)
+//   stencil  (it >= 2) h = z + gamma h_prev or z (f90:1961-1996) and u -= lambda h_prev
+//            (f90:1914-1918) of the previous iteration, then Ah = A h and the partials
+//            of h.Ah                                   (e3x_stencil.hip)
 //   fin_hah  hAh, |hAh| < 1e-14 break, lambda         (f90:1889-1907)
-//   update   u -= lambda h, g -= lambda Ah, z = M^-1 g, partials of z.g (f90:1914-1946)
+//   update   g -= lambda Ah, z = M^-1 g, partials of z.g (f90:1919-1946)
 //   fin_upd  gg, rel_res, history, reset / gamma, stop test (f90:1949-2007)
-//   direction h = z (+ gamma h)                      (f90:1961-1996)
+// The u step and the direction of the last iteration are applied once after the loop
+// (k_pcg_finalize).  The arithmetic of every value is the one of the separate steps.
 // Storage type T of g, h, z, Ah and M^-1 is double, or float in the mixed form; u and
 // every dot product stay double.
 #include "e3x_device.hpp"
@@ -111,8 +114,9 @@ __global__ __launch_bounds__(kFinBlock) void k_hah_finish(const double *partials
     }
 }
 
-// u -= lambda h; g -= lambda Ah (unless the true gradient replaces g this iteration);
-// z = M^-1 g; partials of z.g   (f90:1914-1946)
+// g -= lambda Ah (unless the true gradient replaces g this iteration); z = M^-1 g;
+// partials of z.g (f90:1919-1946).  DO_U: u -= lambda h here (iterations whose true
+// gradient needs the current u); otherwise the next iteration's apply moves u.
 template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
 {
@@ -204,35 +208,50 @@ __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *parti
     }
 }
 
-// h = z + gamma h, or h = z on a reset (f90:1961-1996)
+// After the loop, per right-hand side that ended at iteration K by convergence or the
+// iteration limit: u -= lambda h_K unless iteration K already did (f90:1914-1918), and
+// the direction update that follows the stop test (f90:1961-1996, 2007) into h.  After
+// a curvature break (f90:1899-1902) only h_K is moved into h.  h_K sits in h (K odd) or
+// h_alt (K even).
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_pcg_direction(PcgArgs<T> A)
+__global__ __launch_bounds__(kBlock) void k_pcg_finalize(PcgArgs<T> A, T *h_alt, int replace_every)
 {
-    bool run[NR], rst[NR];
-    double gam[NR];
+    int mode[NR];   // 0 nothing, 1 step + direction, 2 copy h_K
+    bool dou[NR], rst[NR];
+    double lam[NR], gam[NR];
     bool any = false;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const PcgState &S = A.st[r];
-        run[r] = S.stop_iter == 0 || (S.stop_iter == A.it && S.run_dir);
+        const int K = S.stop_iter;
+        mode[r] = 0;
+        if (K >= 1) {
+            if (S.exit_flag == 1 || S.exit_flag == 3) mode[r] = 1;
+            else if (!(K & 1)) mode[r] = 2;
+        }
+        dou[r] = !(replace_every > 0 && K % replace_every == 0);
         rst[r] = S.reset != 0;
+        lam[r] = S.lambda;
         gam[r] = S.gamma;
-        any |= run[r];
+        any |= mode[r] != 0;
     }
     if (!any) return;
     const int64_t ns = A.ns;
     for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            if (!run[r]) continue;
-            const int64_t o = (int64_t)r * 3 * ns + m;
-            if (rst[r]) {
+            if (!mode[r]) continue;
+            const T *hk = (A.st[r].stop_iter & 1) ? A.h : h_alt;
 #pragma unroll
-                for (int c = 0; c < 3; c++) A.h[o + c * ns] = A.z[o + c * ns];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    A.h[o + c * ns] = (T)((double)A.z[o + c * ns] + gam[r] * (double)A.h[o + c * ns]);
+            for (int c = 0; c < 3; c++) {
+                const int64_t o = (int64_t)(3 * r + c) * ns + m;
+                const T hv = hk[o];
+                if (mode[r] == 2) {
+                    A.h[o] = hv;
+                } else {
+                    if (dou[r]) A.u[o] = A.u[o] - lam[r] * (double)hv;
+                    A.h[o] = rst[r] ? A.z[o] : (T)((double)A.z[o] + gam[r] * (double)hv);
+                }
             }
         }
     }
@@ -264,25 +283,37 @@ void init_nr(Solver &s, int kkk, int ldemb)
 }
 
 template <typename T, int NR>
-void iter_nr(Solver &s, int it, bool replace)
+void iter_nr(Solver &s, int it, bool replace, bool prev_replace)
 {
     const int g = (int)vector_grid(s);
     const dim3 vg(g), blk(kBlock), one(1);
-    // 1. Ah = A h and the curvature (f90:1578-1907)
-    (void)launch_apply_hah(s, sizeof(T) == 4, it);
+    // 1. direction and u step of the previous iteration, Ah = A h, the curvature (f90:1578-1907)
+    (void)launch_apply_hah(s, sizeof(T) == 4, it, !prev_replace);
     PcgArgs<T> A = pcg_args<T>(s, it);
     // 2. updates, z and z.g (f90:1914-2007)
     if (!replace) {
-        hipLaunchKernelGGL((k_pcg_update<T, NR, true, true, true>), vg, blk, 0, s.stream, A);
+        hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true>), vg, blk, 0, s.stream, A);
     } else {
-        // reliable update: u first, then the true gradient g = K u + b in FP64, then z
+        // reliable update: u first (with this iteration's direction), then the true
+        // gradient g = K u + b in FP64, then z
+        if constexpr (sizeof(T) == 4) A.h = (it & 1) ? s.h32 : s.h32_alt;
+        else A.h = (it & 1) ? s.h : s.h_alt;
         hipLaunchKernelGGL((k_pcg_update<T, NR, false, true, false>), vg, blk, 0, s.stream, A);
         (void)launch_resid(s);
         hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true>), vg, blk, 0, s.stream, A);
     }
     hipLaunchKernelGGL(k_update_finish<NR>, one, dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.st, s.hist, it);
-    // 3. direction (f90:1961-1996)
-    hipLaunchKernelGGL((k_pcg_direction<T, NR>), vg, blk, 0, s.stream, A);
+}
+
+template <typename T, int NR>
+void finalize_nr(Solver &s, int replace_every)
+{
+    PcgArgs<T> A = pcg_args<T>(s, 0);
+    T *alt;
+    if constexpr (sizeof(T) == 4) alt = s.h32_alt;
+    else alt = s.h_alt;
+    hipLaunchKernelGGL((k_pcg_finalize<T, NR>), dim3((unsigned)vector_grid(s)), dim3(kBlock), 0, s.stream, A, alt,
+                       replace_every);
 }
 
 template <typename T>
@@ -299,15 +330,28 @@ void init_t(Solver &s, int kkk, int ldemb)
 }
 
 template <typename T>
-void iter_t(Solver &s, int it, bool replace)
+void iter_t(Solver &s, int it, bool replace, bool prev_replace)
 {
     switch (s.nrhs) {
-    case 1: iter_nr<T, 1>(s, it, replace); break;
-    case 2: iter_nr<T, 2>(s, it, replace); break;
-    case 3: iter_nr<T, 3>(s, it, replace); break;
-    case 4: iter_nr<T, 4>(s, it, replace); break;
-    case 5: iter_nr<T, 5>(s, it, replace); break;
-    default: iter_nr<T, 6>(s, it, replace); break;
+    case 1: iter_nr<T, 1>(s, it, replace, prev_replace); break;
+    case 2: iter_nr<T, 2>(s, it, replace, prev_replace); break;
+    case 3: iter_nr<T, 3>(s, it, replace, prev_replace); break;
+    case 4: iter_nr<T, 4>(s, it, replace, prev_replace); break;
+    case 5: iter_nr<T, 5>(s, it, replace, prev_replace); break;
+    default: iter_nr<T, 6>(s, it, replace, prev_replace); break;
+    }
+}
+
+template <typename T>
+void finalize_t(Solver &s, int replace_every)
+{
+    switch (s.nrhs) {
+    case 1: finalize_nr<T, 1>(s, replace_every); break;
+    case 2: finalize_nr<T, 2>(s, replace_every); break;
+    case 3: finalize_nr<T, 3>(s, replace_every); break;
+    case 4: finalize_nr<T, 4>(s, replace_every); break;
+    case 5: finalize_nr<T, 5>(s, replace_every); break;
+    default: finalize_nr<T, 6>(s, replace_every); break;
     }
 }
 
@@ -343,10 +387,17 @@ hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
     return hipGetLastError();
 }
 
-hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace)
+hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace, bool prev_replace)
 {
-    if (mixed) iter_t<float>(s, it, replace);
-    else iter_t<double>(s, it, false);
+    if (mixed) iter_t<float>(s, it, replace, prev_replace);
+    else iter_t<double>(s, it, false, false);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_finalize(Solver &s, bool mixed, int replace_every)
+{
+    if (mixed) finalize_t<float>(s, replace_every);
+    else finalize_t<double>(s, 0);
     return hipGetLastError();
 }
 

@@ -36,7 +36,13 @@ struct StencilArgs {
     const double *b;       // kEnergy, kResid
     float *y32;            // kResid output
     double *partials;
-    const PcgState *st;    // kApply, kResid: skip finished right-hand sides
+    const PcgState *st;    // kApply, kApplyFused, kResid: skip finished right-hand sides
+    // kApplyFused: x is formed as z + gamma hold (z on a reset) and written to hnew;
+    // with fuse_u, u -= lambda hold (gamma, reset, lambda of the previous iteration)
+    const T *z, *hold;
+    T *hnew;
+    double *u;
+    int fuse_u;
 };
 
 __device__ __forceinline__ int pmod(int v, int n) { v %= n; return v < 0 ? v + n : v; }
@@ -51,8 +57,11 @@ template <typename T, int NR, int MODE>
 __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint16_t *__restrict__ pix,
                                              const T *__restrict__ dk, const T *__restrict__ sten,
                                              const T *__restrict__ x, T *__restrict__ y,
-                                             const double *__restrict__ bvec, float *__restrict__ y32)
+                                             const double *__restrict__ bvec, float *__restrict__ y32,
+                                             const T *__restrict__ zv, const T *__restrict__ hold,
+                                             T *__restrict__ hnew, double *__restrict__ u)
 {
+    constexpr bool FUSED = MODE == kApplyFused;
     __shared__ T hs[3 * HN];
     __shared__ int ps[VN];
     __shared__ int cnt[kBlock / 64];
@@ -67,27 +76,70 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // Interior tiles index without wrapping; boundary tiles wrap each coordinate.
     const bool interior = ox >= 1 && oy >= 1 && oz >= 1 && ox + TX + 1 <= nx && oy + TY + 1 <= ny && oz + TZ + 1 <= nz;
     const int corner = (oz - 1) * nxy + (oy - 1) * nx + (ox - 1);
-    auto stage = [&](int r) {
-        const T *x0 = x + (int64_t)(3 * r) * ns, *x1 = x0 + ns, *x2 = x1 + ns;
-        if (interior) {  // unrolled: all loads of a thread in flight together
+    unsigned live = (1u << NR) - 1u;  // right-hand sides to compute
+    if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kResid) {
+        live = 0u;
 #pragma unroll
-            for (int it = 0; it < (HN + kBlock - 1) / kBlock; it++) {
-                const int q = tid + it * kBlock;
-                if (q < HN) {
-                    const int g = corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX;
-                    hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+        for (int r = 0; r < NR; r++)
+            if (A.st[r].stop_iter == 0) live |= 1u << r;
+    }
+    auto stage = [&](int r) {
+        const int64_t o = (int64_t)(3 * r) * ns;
+        if constexpr (FUSED) {
+            // this iteration's direction, h = z + gamma h_prev or h = z on a reset
+            // (f90:1961-1996); the block's own nodes also store it and take the previous
+            // step into u, u -= lambda h_prev (f90:1914-1918)
+            const double gam = A.st[r].gamma, lam = A.st[r].lambda;
+            const bool rst = A.st[r].reset != 0, run = (live >> r) & 1u, dou = run && A.fuse_u;
+            auto put = [&](int q, int g) {
+                const int hx = q % HX, hy = (q / HX) % HY, hz = q / (HX * HY);
+                const bool own = run && hx >= 1 && hx <= TX && hy >= 1 && hy <= TY && hz >= 1 && hz <= TZ &&
+                                 ox + hx - 1 < nx && oy + hy - 1 < ny && oz + hz - 1 < nz;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int64_t i = o + c * ns + g;
+                    const T zc = zv[i], hc = hold[i];
+                    const T hv = rst ? zc : (T)((double)zc + gam * (double)hc);
+                    hs[c * HN + q] = hv;
+                    if (own) {
+                        hnew[i] = hv;
+                        if (dou) u[i] = u[i] - lam * (double)hc;
+                    }
                 }
+            };
+            if (interior) {
+#pragma unroll
+                for (int it = 0; it < (HN + kBlock - 1) / kBlock; it++) {
+                    const int q = tid + it * kBlock;
+                    if (q < HN) put(q, corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX);
+                }
+            } else {
+                for (int q = tid; q < HN; q += kBlock)
+                    put(q, pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
+                               nxy * pmod(oz + q / (HX * HY) - 1, nz));
             }
         } else {
-            for (int q = tid; q < HN; q += kBlock) {
-                const int g = pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
-                              nxy * pmod(oz + q / (HX * HY) - 1, nz);
-                hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+            const T *x0 = x + o, *x1 = x0 + ns, *x2 = x1 + ns;
+            if (interior) {  // unrolled: all loads of a thread in flight together
+#pragma unroll
+                for (int it = 0; it < (HN + kBlock - 1) / kBlock; it++) {
+                    const int q = tid + it * kBlock;
+                    if (q < HN) {
+                        const int g = corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX;
+                        hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+                    }
+                }
+            } else {
+                for (int q = tid; q < HN; q += kBlock) {
+                    const int g = pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
+                                  nxy * pmod(oz + q / (HX * HY) - 1, nz);
+                    hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+                }
             }
         }
     };
 
-    stage(0);  // issued before the state check below, so its latency overlaps it
+    stage(0);  // issued before the early exit below, so its latency overlaps the state loads
     if (interior) {
 #pragma unroll
         for (int it = 0; it < (VN + kBlock - 1) / kBlock; it++) {
@@ -99,14 +151,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             ps[q] = pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
                         nxy * pmod(oz + q / (VX * VY) - 1, nz)];
     }
-    unsigned live = (1u << NR) - 1u;  // right-hand sides to compute
-    if constexpr (MODE == kApply || MODE == kResid) {
-        live = 0u;
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-            if (A.st[r].stop_iter == 0) live |= 1u << r;
-        if (!live) return;  // uniform over the block
-    }
+    if (!live) return;  // uniform over the block
     __syncthreads();
 
     // Stable compaction: homogeneous nodes first, then mixed ones.
@@ -227,7 +272,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 
         if (active) {
             const T acc[3] = {acc0, acc1, acc2};
-            if constexpr (MODE == kApply) {
+            if constexpr (MODE == kApply || MODE == kApplyFused) {
                 double pr = 0.0;
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
@@ -260,7 +305,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             }
         }
     }
-    if constexpr (MODE == kApply || MODE == kEnergy) write_wave_partials(part, A.partials);
+    if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kEnergy) write_wave_partials(part, A.partials);
 }
 #undef E3X_NB
 
@@ -269,7 +314,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 template <typename T, int NR, int MODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_stencil(StencilArgs<T> A)
 {
-    stencil_body<T, NR, MODE>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32);
+    stencil_body<T, NR, MODE>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold, A.hnew, A.u);
 }
 
 template <int NR>
@@ -345,17 +390,44 @@ hipError_t launch_apply_plain(Solver &s, const double *x, double *y)
     return hipGetLastError();
 }
 
-hipError_t launch_apply_hah(Solver &s, bool mixed, int it)
+// Iteration 1 applies K to h as it stands; iteration it >= 2 forms its direction from
+// z and the previous one (h_alt / h by parity) while staging.
+template <typename T>
+void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T *ah, int it, bool fuse_u)
 {
-    if (mixed) dispatch_stencil<float, kApply>(s, stencil_args<float>(s, s.dk32, s.sten32, s.h32, s.ah32));
-    else dispatch_stencil<double, kApply>(s, stencil_args<double>(s, s.dk, s.sten, s.h, s.ah));
+    if (it <= 1) {
+        dispatch_stencil<T, kApply>(s, stencil_args<T>(s, dk, sten, hbuf[1], ah));
+    } else {
+        StencilArgs<T> A = stencil_args<T>(s, dk, sten, nullptr, ah);
+        A.z = z; A.hold = hbuf[(it - 1) & 1]; A.hnew = hbuf[it & 1];
+        A.u = s.u; A.fuse_u = fuse_u ? 1 : 0;
+        dispatch_stencil<T, kApplyFused>(s, A);
+    }
+}
+
+hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u)
+{
+    if (mixed) {
+        float *hb[2] = {s.h32_alt, s.h32};
+        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, it, fuse_u);
+    } else {
+        double *hb[2] = {s.h_alt, s.h};
+        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, it, fuse_u);
+    }
     return launch_hah_finish(s, (int)stencil_parts(s), it);
 }
 
+// The fused apply of an even iteration, as the PCG loop runs it (u and the state are
+// left to the caller: with lambda = gamma = 0 it leaves u unchanged and sets h = z).
 hipError_t launch_apply_only(Solver &s, bool mixed)
 {
-    if (mixed) dispatch_stencil<float, kApply>(s, stencil_args<float>(s, s.dk32, s.sten32, s.h32, s.ah32));
-    else dispatch_stencil<double, kApply>(s, stencil_args<double>(s, s.dk, s.sten, s.h, s.ah));
+    if (mixed) {
+        float *hb[2] = {s.h32_alt, s.h32};
+        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, 2, true);
+    } else {
+        double *hb[2] = {s.h_alt, s.h};
+        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, 2, true);
+    }
     return hipGetLastError();
 }
 

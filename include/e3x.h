@@ -171,10 +171,12 @@ int e3x_last_timing(e3x_solver *s, double *precond_ms, double *pcg_ms);
 
 /*
  * Measurement hook (no counterpart in the reference): replay `reps` launches of the
- * PCG's stencil-apply kernel (Ah = K h with the h.Ah block partials, FP64 or the mixed
- * form's FP32) on the solver's stream, bracketed by HIP events on that stream, and
- * return the average launch time in ms.  The PCG state is saved and restored; h, Ah
- * and the partials are overwritten.  Needs e3x_femat first.
+ * PCG's stencil-apply kernel in the form the loop runs from its second iteration on
+ * (direction h = z + gamma h_prev, u -= lambda h_prev, Ah = K h with the h.Ah
+ * partials; FP64 or the mixed form's FP32) on the solver's stream, bracketed by HIP
+ * events on that stream, and return the average launch time in ms.  It runs with
+ * lambda = gamma = 0, so u keeps its values; the PCG state is saved and restored; the
+ * second direction buffer, Ah and the partials are overwritten.  Needs e3x_femat first.
  */
 int e3x_time_apply(e3x_solver *s, int precision, int reps, double *avg_ms);
 

@@ -152,13 +152,16 @@ def test_dembx_matches_oracle(shape, ngrain):
         assert abs(gg0[0] - gg0_o) < 1e-11 * gg0_o
         d = s.dembx(gg0)
         u = s.get(L.FIELD_U)
-        gb = s.get(L.FIELD_GB)
+        h = s.get(L.FIELD_H)
     assert d["exit"][0] == L.PCG_CONVERGED == r_o["status"]
     assert abs(int(d["lstep"][0]) - r_o["lstep"]) <= max(1, int(0.01 * r_o["lstep"]))
     if d["lstep"][0] == r_o["lstep"]:
         assert _rel(u, r_o["u"]) < 1e-10
         assert _rel(d["history"][0], r_o["history"]) < 1e-8
         assert abs(d["gg"][0] - r_o["gg"]) < 1e-8 * r_o["gg"]
+        # the direction after the stop test (f90:1961-1996, 2007): h is of the size of the
+        # final residual, so its relative agreement is looser than u's
+        assert _rel(h, r_o["h"]) < 1e-5
 
 
 def test_full_solve_matches_oracle_config1_shape():
@@ -201,22 +204,44 @@ def test_hah_break_on_all_pore_grid():
         assert np.array_equal(s.get(L.FIELD_U), u0)
 
 
-def test_maxiter_and_history_prefix():
+@pytest.mark.parametrize("ldemb", [5, 6])
+def test_maxiter_and_history_prefix(ldemb):
+    """Stop by the iteration limit after an odd and an even number of iterations (the
+    last direction is formed in either of the two direction buffers)."""
     shape = (12, 12, 12)
     pix, rod = voronoi_case(shape, 5, seed=400)
     e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
-    r_o, _ = _oracle_pcg(shape, pix, rod, e6, ldemb=5)
+    r_o, _ = _oracle_pcg(shape, pix, rod, e6, ldemb=ldemb)
     assert r_o["status"] == oc.OC_MAXITER
     with Elas3D(shape, 6) as s:
         s.ppixel(pix, rod)
         s.femat(C11, C12, C44, e6)
         s.affine_guess()
         _, gg0 = s.energy()
-        d = s.dembx(gg0, ldemb=5)
+        d = s.dembx(gg0, ldemb=ldemb)
         u = s.get(L.FIELD_U)
-    assert d["exit"][0] == L.PCG_MAXITER and d["lstep"][0] == 5
+        h = s.get(L.FIELD_H)
+    assert d["exit"][0] == L.PCG_MAXITER and d["lstep"][0] == ldemb
     assert _rel(d["history"][0], r_o["history"]) < 1e-10
     assert _rel(u, r_o["u"]) < 1e-12
+    assert _rel(h, r_o["h"]) < 1e-10
+
+
+@pytest.mark.parametrize("replace_every", [0, 7, 1000])
+def test_mixed_precision_against_fp64_oracle(replace_every):
+    """FP32 stencil/vectors/preconditioner with FP64 u and dots, true-residual
+    replacement every `replace_every` iterations (0: never): the converged solution
+    agrees with the FP64 oracle to the level of the stop tolerance."""
+    shape = (16, 14, 12)
+    pix, rod = voronoi_case(shape, 7, seed=600)
+    e6 = [oc.AML_F32, 0, 0, 0, 0.3e-3, 0]
+    ro = oc.solve(shape, 8, pix, rod, C11, C12, C44, e6)
+    res = solve(pix, rod, shape, 8, C11, C12, C44, e6, precision=L.PREC_MIXED, replace_every=replace_every)
+    assert res.exit[0] == L.PCG_CONVERGED
+    assert abs(int(res.lstep[0]) - ro["lstep"]) <= max(2, int(0.05 * ro["lstep"]))
+    assert abs(res.utot[0] - ro["utot"]) < 1e-6 * abs(ro["utot"])
+    assert _rel(res.stress6[0], ro["stress6"]) < 1e-5
+    assert _rel(res.strain6[0], ro["strain6"]) < 1e-12
 
 
 def test_multi_rhs_equals_single_solves():
