@@ -94,6 +94,35 @@ def cpu_baseline(w, c11, c12, c44, e6, iters):
                        f"in {dt:.1f} s: oracle C restatement of dembx_OpenMP, OpenMP")
 
 
+def profile_run(n, zener):
+    """Short fixed workload for hardware-counter passes: the 200^3 problem after a
+    3-iteration PCG (so z and h hold PCG values), 5 launches of the fused apply kernel
+    (FP64, then the mixed form's FP32), then a fill of 3*ns doubles (3*ns*8 bytes
+    written) and 3 elementwise copies (3*ns*8 bytes read and written each), whose
+    counter readings calibrate the bytes."""
+    import torch
+
+    w = configs.build("cfg2", n=n)
+    c11, c12, c44 = configs.materials("cfg2", zener=zener)
+    ns = int(np.prod(w["shape"]))
+    with Elas3D(w["shape"], w["nphase"]) as s:
+        s.ppixel(w["pix"], w["rod"])
+        s.femat(c11, c12, c44, configs.exx_strain())
+        s.affine_guess()
+        _, gg0 = s.energy()
+        s.dembx(gg0, ldemb=3, want_history=False)
+        t64 = s.time_apply(precision=L.PREC_FP64, reps=5)
+        s.dembx(gg0, ldemb=3, precision=L.PREC_MIXED, want_history=False)
+        t32 = s.time_apply(precision=L.PREC_MIXED, reps=5)
+    x = torch.ones(3 * ns, dtype=torch.float64, device="cuda")   # fill: writes 24*ns bytes
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)                                   # reads and writes 24*ns bytes
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": "apply", "ns": ns, "apply_us_fp64": t64 * 1e3, "apply_us_mixed": t32 * 1e3,
+                      "copy_bytes_each_way": 24 * ns}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +133,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short fixed run of the apply kernel and a calibration copy, for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the solve on a single MI355X "
@@ -111,6 +142,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.profile:
+        profile_run(args.n, args.zener)
+        return
     w = configs.build("cfg2", n=args.n)
     c11, c12, c44 = configs.materials("cfg2", zener=args.zener)
     e6 = configs.exx_strain()

@@ -82,18 +82,36 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *partials
     }
 }
 
-// Per-wave partials, layout [q][wave of the grid]: no block barrier, so a wave that is
-// done leaves at once instead of waiting for the slowest wave of its block.  Called by
-// every lane of the wave (all lanes active).
+// Block partials, layout [q][block], without a block barrier: each wave leaves its sum
+// in LDS and counts itself in; the last wave to arrive adds the four sums in wave order
+// (a fixed order, whichever wave is last) and stores the partial.  A finished wave
+// leaves at once instead of waiting for the slowest wave of its block.  `arrived` is a
+// block-shared counter set to 0 before a __syncthreads() that precedes this call; every
+// lane of every wave calls it (all lanes active).
 template <int NV>
-__device__ __forceinline__ void write_wave_partials(double (&v)[NV], double *partials)
+__device__ __forceinline__ void write_partials_nowait(double (&v)[NV], double *partials, unsigned *arrived)
 {
-    const int nw = gridDim.x * (kBlock / 64);
-    const int wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    constexpr int NW = kBlock / 64;
+    __shared__ double sh[NV][NW];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double t[NV];
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-        const double t = wave_sum(v[q]);
-        if ((threadIdx.x & 63) == 63) partials[(int64_t)q * nw + wid] = t;
+    for (int q = 0; q < NV; q++) t[q] = wave_sum(v[q]);
+    unsigned prev = 0;
+    if (l == 63) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) sh[q][w] = t[q];
+        prev = __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    prev = __builtin_amdgcn_readlane(prev, 63);
+    if (prev == NW - 1 && l == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double tot = sh[q][0];
+#pragma unroll
+            for (int ww = 1; ww < NW; ww++) tot += sh[q][ww];
+            partials[(int64_t)q * gridDim.x + blockIdx.x] = tot;
+        }
     }
 }
 

@@ -110,7 +110,7 @@ hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s
 hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st);
 
 int64_t stencil_blocks(const Solver &s);
-int64_t stencil_parts(const Solver &s);  // partials written per stencil launch (one per wave)
+int64_t stencil_parts(const Solver &s);  // partials written per stencil launch (one per block)
 inline int64_t vector_blocks(const Solver &s) { return (s.ns + kBlock - 1) / kBlock; }
 // streaming kernels: grid-stride with at most kVecGridMax blocks (8 waves per SIMD)
 constexpr int kVecGridMax = 2048;

@@ -66,6 +66,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     __shared__ int ps[VN];
     __shared__ int cnt[kBlock / 64];
     __shared__ short order[kBlock];
+    __shared__ unsigned arrived;  // waves done, for the block partials
 
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = bid % A.ntx, ty = (bid / A.ntx) % A.nty, tz = bid / (A.ntx * A.nty);
@@ -73,6 +74,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     const int64_t ns = A.ns;
     const int tid = threadIdx.x;
+    if (tid == 0) arrived = 0u;  // ordered before use by the barriers below
     // Interior tiles index without wrapping; boundary tiles wrap each coordinate.
     const bool interior = ox >= 1 && oy >= 1 && oz >= 1 && ox + TX + 1 <= nx && oy + TY + 1 <= ny && oz + TZ + 1 <= nz;
     const int corner = (oz - 1) * nxy + (oy - 1) * nx + (ox - 1);
@@ -305,7 +307,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             }
         }
     }
-    if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kEnergy) write_wave_partials(part, A.partials);
+    if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kEnergy) write_partials_nowait(part, A.partials, &arrived);
 }
 #undef E3X_NB
 
@@ -362,7 +364,7 @@ int64_t stencil_blocks(const Solver &s)
     return (int64_t)((s.nx + TX - 1) / TX) * ((s.ny + TY - 1) / TY) * ((s.nz + TZ - 1) / TZ);
 }
 
-int64_t stencil_parts(const Solver &s) { return stencil_blocks(s) * (kBlock / 64); }
+int64_t stencil_parts(const Solver &s) { return stencil_blocks(s); }
 
 hipError_t launch_energy(Solver &s)
 {
