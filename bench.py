@@ -37,6 +37,22 @@ HBM_PEAK_GBS = 8000.0
 APPLY_BYTES_PER_NODE = {L.PREC_FP64: 6 * 3 * 8 + 2, L.PREC_MIXED: 4 * 3 * 4 + 2 * 3 * 8 + 2}
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "stencil_apply_cfg2_200_pmc.txt")
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the apply kernel on the 200^3 A=2 workload, from the
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (None if absent)."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def solve_resident(s, c11, c12, c44, e6, precision):
     """One step on the resident microstructure: the main program's sequence
     (src/elas3dxtal_pcg.f90:417-519)."""
@@ -158,8 +174,10 @@ def main():
     apply_s = fp64["apply_ms"] * 1e-3
     alg_bytes = APPLY_BYTES_PER_NODE[L.PREC_FP64] * ns
     achieved = alg_bytes / apply_s / 1e9
+    default_workload = args.n == 200 and args.zener == 2.0
+    traffic = measured_traffic("traffic_fp64_bytes_per_launch") if default_workload else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel="k_stencil<double,1,kApplyFused> (h = z + gamma h_prev, u -= lambda h_prev, "
                            "Ah = K h, h.Ah partials)",
                     avg_launch_us=round(fp64["apply_ms"] * 1e3, 2),
