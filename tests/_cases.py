@@ -1,0 +1,36 @@
+"""Seeded test microstructures shared by the oracle pins and the GPU parity tests."""
+import numpy as np
+
+import _indep as ind
+
+
+def voronoi_case(shape, ngrain, seed, pore_frac=0.02):
+    """Periodic Voronoi grains (nearest seed, minimum image) + random single-voxel pores
+    (phase ngrain+1).  Returns pix (1-based labels, node order m = i + nx(j + ny k)) and
+    Rodrigues vectors of the grains."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    dims = np.array([nx, ny, nz], float)
+    seeds = rng.random((ngrain, 3)) * dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pts = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1) + 0.5
+    best = np.full(pts.shape[0], np.inf)
+    lab = np.zeros(pts.shape[0], np.int32)
+    for g in range(ngrain):
+        d = np.abs(pts - seeds[g])
+        d = np.minimum(d, dims - d)
+        d2 = np.sum(d * d, axis=1)
+        upd = d2 < best
+        best[upd] = d2[upd]
+        lab[upd] = g + 1
+    npore = int(round(pore_frac * pts.shape[0]))
+    if npore:
+        lab[rng.choice(pts.shape[0], npore, replace=False)] = ngrain + 1
+    rod = ind.random_rodrigues(rng, ngrain)
+    return lab, rod
+
+
+# A case whose PCG resets its direction (f90:1961-1981): cubic grains with negative
+# pore moduli (K, G) = (-160, 10) GPa make the stiffness indefinite, and the
+# preconditioned residual jumps by more than 10x several times before it converges.
+RESET_CASE = dict(shape=(6, 6, 6), ngrain=2, seed=2, pore_frac=0.1, phasemod=(-160e9, 10e9))

@@ -185,3 +185,60 @@ def random_rodrigues(rng, n):
     qy = np.sqrt(u1) * np.sin(2 * np.pi * u3)
     qz = np.sqrt(u1) * np.cos(2 * np.pi * u3)
     return np.stack([qx / qw, qy / qw, qz / qw], axis=1)
+
+
+def block_jacobi_dense(kg, ns):
+    """Inverses of the 3x3 diagonal blocks of a dense K (dof c*ns + m), numpy; a block
+    with |det| < 1e-14 gets a zero inverse.  Returns (ns, 3, 3)."""
+    out = np.zeros((ns, 3, 3))
+    for m in range(ns):
+        idx = [m, ns + m, 2 * ns + m]
+        blk = kg[np.ix_(idx, idx)]
+        if abs(np.linalg.det(blk)) >= 1e-14:
+            out[m] = np.linalg.inv(blk)
+    return out
+
+
+def pcg_reference_dense(kg, minv, u, g, gg0, tol, ldemb):
+    """The PCG of dembx_OpenMP written from its text (src/elas3dxtal_pcg.f90:1527-2007)
+    over a dense matrix: z = M^-1 g, h = z; per step hAh (break if |hAh| < 1e-14),
+    lambda = gg/hAh, u -= lambda h, g -= lambda Ah, z = M^-1 g, rel_res = sqrt(z.g/gg0);
+    if rel_res > 10 rel_res_prev (step > 1) h = z and count a reset, else
+    h = z + (gg/gglast) h; then stop when rel_res < tol.
+    Returns dict(u, g, h, history, lstep, resets, status) with status 1 converged,
+    2 curvature break, 3 iteration limit."""
+    ns = kg.shape[0] // 3
+    u, g = u.copy(), g.copy()
+
+    def prec(v):
+        w = v.reshape(3, ns).T                     # (ns, 3)
+        return np.einsum("mij,mj->mi", minv, w).T.ravel()
+
+    z = prec(g)
+    h = z.copy()
+    gg = float(z @ g)
+    rel_prev = np.sqrt(gg / gg0)
+    resets, hist, status, step = 0, [], 3, 0
+    for step in range(1, ldemb + 1):
+        ah = kg @ h
+        hah = float(h @ ah)
+        if abs(hah) < 1e-14:
+            status = 2
+            break
+        lam = gg / hah
+        u -= lam * h
+        g -= lam * ah
+        z = prec(g)
+        gglast, gg = gg, float(z @ g)
+        rel = np.sqrt(gg / gg0)
+        hist.append(rel)
+        if rel > 10.0 * rel_prev and step > 1:
+            h = z.copy()
+            resets += 1
+        else:
+            h = z + (gg / gglast) * h
+        rel_prev = rel
+        if rel < tol:
+            status = 1
+            break
+    return dict(u=u, g=g, h=h, history=np.array(hist), lstep=step, resets=resets, status=status)

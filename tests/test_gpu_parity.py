@@ -9,6 +9,7 @@ import pytest
 
 import _indep as ind
 import _oracle as oc
+from _cases import RESET_CASE, voronoi_case
 from elas3dxtal import _lib as L
 from elas3dxtal.solver import Elas3D, solve
 
@@ -20,30 +21,6 @@ C11, C12, C44 = 246.5e9, 147.3e9, 124.7e9
 def _rel(a, b):
     a, b = np.asarray(a, float), np.asarray(b, float)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
-
-
-def voronoi_case(shape, ngrain, seed, pore_frac=0.02):
-    """Periodic Voronoi grains (nearest seed, minimum image) + random single-voxel pores."""
-    rng = np.random.default_rng(seed)
-    nx, ny, nz = shape
-    dims = np.array([nx, ny, nz], float)
-    seeds = rng.random((ngrain, 3)) * dims
-    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    pts = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1) + 0.5
-    best = np.full(pts.shape[0], np.inf)
-    lab = np.zeros(pts.shape[0], np.int32)
-    for g in range(ngrain):
-        d = np.abs(pts - seeds[g])
-        d = np.minimum(d, dims - d)
-        d2 = np.sum(d * d, axis=1)
-        upd = d2 < best
-        best[upd] = d2[upd]
-        lab[upd] = g + 1
-    npore = int(round(pore_frac * pts.shape[0]))
-    if npore:
-        lab[rng.choice(pts.shape[0], npore, replace=False)] = ngrain + 1
-    rod = ind.random_rodrigues(rng, ngrain)
-    return lab, rod
 
 
 def oracle_tables(nphase, rod, pm=(0.0, 0.0)):
@@ -188,6 +165,30 @@ def test_full_solve_matches_oracle_config1_shape():
     sf_o, vm_o = oc.fullfield(shape, pix, cm_o, e6, ro["u"])
     assert _rel(sf, sf_o) < 1e-9
     assert _rel(vm, vm_o) < 1e-9
+
+
+def test_divergence_reset_matches_oracle():
+    """An indefinite case (negative pore moduli) on which the PCG resets its direction
+    several times (f90:1961-1981): same resets, iterations and history as the oracle."""
+    rc = RESET_CASE
+    shape = rc["shape"]
+    pix, rod = voronoi_case(shape, rc["ngrain"], rc["seed"], rc["pore_frac"])
+    nphase = rc["ngrain"] + 1
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    ro = oc.solve(shape, nphase, pix, rod, C11, C12, C44, e6, phasemod=rc["phasemod"])
+    res, s = solve(pix, rod, shape, nphase, C11, C12, C44, e6, phasemod=rc["phasemod"], keep=True)
+    try:
+        u = s.get(L.FIELD_U)
+    finally:
+        s.close()
+    assert ro["reset_count"] >= 2
+    assert res.reset_count[0] == ro["reset_count"]
+    assert res.exit[0] == L.PCG_CONVERGED
+    assert res.lstep[0] == ro["lstep"]
+    # indefinite operator: summation-order rounding grows to ~1e-6 (see the oracle pin)
+    assert _rel(res.history[0], ro["history"]) < 1e-5
+    assert abs(res.utot[0] - ro["utot"]) < 1e-8 * abs(ro["utot"])
+    assert _rel(u, ro["u"]) < 1e-6
 
 
 def test_hah_break_on_all_pore_grid():
