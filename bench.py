@@ -12,10 +12,13 @@ affine guess, energy, block-Jacobi PCG to convergence, energy, averaged stress.
 value = FP64 PCG DOF-updates/s = 3 * ns * Lstep / t_pcg (t_pcg: GPU time of the
 PCG loop, HIP events on the solver's stream), summed over the K timed steps.
 Also reported: solves/s and per-solve latency percentiles (host wall clock), the
-same for the mixed-precision form, the roofline of the stencil-apply kernel, and
-the CPU baseline (the oracle's C/OpenMP restatement of the reference on this host).
+same for the mixed-precision form, the six unit load cases of the effective stiffness
+as six right-hand sides of one PCG (FP64 and mixed; each right-hand side counts as a
+solve), the roofline of the stencil-apply kernel, and the CPU baseline (the oracle's
+C/OpenMP restatement of the reference on this host).
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -86,6 +89,28 @@ def run_gpu(w, c11, c12, c44, e6, steps, warmup, precision):
         s.close()
 
 
+def run_six(w, c11, c12, c44, steps, warmup, precision):
+    """The six unit load cases of the effective stiffness as six right-hand sides of one
+    PCG (e3x_effective_stiffness): one step = femat .. averages for all six, giving C_eff."""
+    s = Elas3D(w["shape"], w["nphase"], nrhs=6)
+    try:
+        s.ppixel(w["pix"], w["rod"])
+        for _ in range(warmup):
+            s.effective_stiffness(c11, c12, c44, precision=precision)
+        walls, pcg_ms, lsteps, ce = [], [], [], None
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            ce, lstep, ex = s.effective_stiffness(c11, c12, c44, precision=precision)
+            walls.append(time.perf_counter() - t0)
+            pre, pcg = C.c_double(), C.c_double()
+            L.check(s._lib.e3x_last_timing(s._h, C.byref(pre), C.byref(pcg)))
+            pcg_ms.append(pcg.value)
+            lsteps.append(int(np.sum(lstep)))
+        return dict(walls=np.array(walls), pcg_ms=pcg_ms, lsteps=lsteps, ceff=ce, lstep_max=int(np.max(lstep)))
+    finally:
+        s.close()
+
+
 def cpu_baseline(w, c11, c12, c44, e6, iters):
     """The oracle (C/OpenMP restatement of the reference) on this host: `iters` PCG
     iterations on the same 200^3 workload, timed around oc_dembx only."""
@@ -149,6 +174,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-six", action="store_true", help="skip the six-load-case (effective stiffness) runs")
+    ap.add_argument("--six-steps", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run of the apply kernel and a calibration copy, for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -222,6 +249,23 @@ def main():
             "stress_xx_rel_err_vs_fp64": abs(mx["stress_xx"] - fp64["stress_xx"]) / abs(fp64["stress_xx"]),
             "apply_us": round(mx["apply_ms"] * 1e3, 2),
         }
+    if not args.no_six:
+        six = {}
+        for name, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+            r = run_six(w, c11, c12, c44, args.six_steps, 1, prec)
+            six[name] = {
+                "solves_per_s": float(6.0 / r["walls"].mean()),
+                "dof_updates_per_s": 3.0 * ns * sum(r["lsteps"]) / (sum(r["pcg_ms"]) * 1e-3),
+                "batch_ms_p50": float(np.percentile(r["walls"], 50) * 1e3),
+                "batch_ms_p99": float(np.percentile(r["walls"], 99) * 1e3),
+                "lstep_max": r["lstep_max"],
+                "ceff_GPa": np.round(r["ceff"] / 1e9, 4).tolist(),
+            }
+            six[name + "_ceff"] = r["ceff"]
+        ce64, ce32 = six.pop("fp64_ceff"), six.pop("mixed_ceff")
+        six["ceff_mixed_rel_err_vs_fp64"] = float(np.max(np.abs(ce32 - ce64)) / np.max(np.abs(ce64)))
+        six["mixed_six_vs_fp64_single_solves_ratio"] = six["mixed"]["solves_per_s"] / out["solves_per_s"]
+        out["six_rhs"] = six
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, c11, c12, c44, e6, args.cpu_iters)
     else:

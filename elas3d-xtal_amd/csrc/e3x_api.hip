@@ -427,6 +427,58 @@ int e3x_stress_fullfield(e3x_solver *h, int rhs, double *stress, double *vm)
     return E3X_OK;
 }
 
+int e3x_effective_stiffness(e3x_solver *h, double c11, double c12, double c44, const double *phasemod,
+                            int flag_m, double magnitude, int precision, int replace_every, float tol,
+                            int ldemb, double *C_eff, int *lstep, int *exit_flag)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (s.nrhs != 6) return fail(E3X_EINVAL, "e3x_effective_stiffness needs a handle with nrhs = 6");
+    if (!C_eff) return fail(E3X_EINVAL, "C_eff is NULL");
+    if (!(magnitude != 0.0) || !std::isfinite(magnitude)) return fail(E3X_EINVAL, "magnitude must be finite and nonzero");
+    double e6[36] = {};
+    for (int k = 0; k < 6; k++) e6[6 * k + k] = k < 3 ? magnitude : 0.5 * magnitude;
+    if (int rc = e3x_femat(h, c11, c12, c44, phasemod, flag_m, e6, nullptr)) return rc;
+    if (int rc = e3x_affine_guess(h)) return rc;
+    double ut[6], gg0[6], gg[6];
+    if (int rc = e3x_energy(h, ut, gg0)) return rc;
+    int L[6], ex[6], resets[6];
+    if (int rc = e3x_dembx_ex(h, precision, replace_every, gg0, tol, ldemb, 1, L, gg, nullptr, ex, resets))
+        return rc;
+    double sig[36], eps[36];
+    if (int rc = e3x_stress(h, sig, eps)) return rc;
+    // columns k of S and E are load case k: C E = S, solved as E^T C^T = S^T by
+    // Gauss-Jordan elimination with partial pivoting
+    double a[6][12];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            a[i][j] = eps[6 * i + j];       // E^T[i][j] = eps of case i, component j
+            a[i][6 + j] = sig[6 * i + j];   // S^T[i][j] = sigma of case i, component j
+        }
+    for (int c = 0; c < 6; c++) {
+        int p = c;
+        for (int r = c + 1; r < 6; r++)
+            if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        if (a[p][c] == 0.0) return fail(E3X_EINVAL, "averaged strains of the six load cases are singular");
+        if (p != c)
+            for (int j = 0; j < 12; j++) std::swap(a[p][j], a[c][j]);
+        const double inv = 1.0 / a[c][c];
+        for (int j = 0; j < 12; j++) a[c][j] *= inv;
+        for (int r = 0; r < 6; r++) {
+            if (r == c || a[r][c] == 0.0) continue;
+            const double f = a[r][c];
+            for (int j = 0; j < 12; j++) a[r][j] -= f * a[c][j];
+        }
+    }
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) C_eff[6 * i + j] = a[j][6 + i];  // C = (C^T)^T
+    for (int k = 0; k < 6; k++) {
+        if (lstep) lstep[k] = L[k];
+        if (exit_flag) exit_flag[k] = ex[k];
+    }
+    return E3X_OK;
+}
+
 int e3x_get_field(e3x_solver *h, int which, int rhs, double *host)
 {
     if (int rc = check_handle(h)) return rc;

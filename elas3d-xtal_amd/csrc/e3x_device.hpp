@@ -82,14 +82,15 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *partials
     }
 }
 
-// Block partials, layout [q][block], without a block barrier: each wave leaves its sum
+// Block partials at partials[slot[q]] (none where slot[q] < 0), without a block barrier: each wave leaves its sum
 // in LDS and counts itself in; the last wave to arrive adds the four sums in wave order
 // (a fixed order, whichever wave is last) and stores the partial.  A finished wave
 // leaves at once instead of waiting for the slowest wave of its block.  `arrived` is a
 // block-shared counter set to 0 before a __syncthreads() that precedes this call; every
 // lane of every wave calls it (all lanes active).
 template <int NV>
-__device__ __forceinline__ void write_partials_nowait(double (&v)[NV], double *partials, unsigned *arrived)
+__device__ __forceinline__ void write_partials_nowait(double (&v)[NV], double *partials, const int64_t (&slot)[NV],
+                                                      unsigned *arrived)
 {
     constexpr int NW = kBlock / 64;
     __shared__ double sh[NV][NW];
@@ -110,7 +111,7 @@ __device__ __forceinline__ void write_partials_nowait(double (&v)[NV], double *p
             double tot = sh[q][0];
 #pragma unroll
             for (int ww = 1; ww < NW; ww++) tot += sh[q][ww];
-            partials[(int64_t)q * gridDim.x + blockIdx.x] = tot;
+            if (slot[q] >= 0) partials[slot[q]] = tot;
         }
     }
 }

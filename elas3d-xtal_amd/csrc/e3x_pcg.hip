@@ -90,27 +90,22 @@ __global__ __launch_bounds__(kFinBlock) void k_pcg_init_finish(const double *par
 }
 
 // hAh, the curvature guard and lambda (f90:1889-1907)
-template <int NR>
+// hAh, the curvature guard and lambda (f90:1889-1907); block r serves right-hand side r
 __global__ __launch_bounds__(kFinBlock) void k_hah_finish(const double *partials, int nparts, PcgState *st, int it)
 {
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; r++) any |= st[r].stop_iter == 0;
-    if (!any) return;
-    double tot[NR];
-    sum_partials<NR>(partials, nparts, tot);
+    const int r = blockIdx.x;
+    PcgState &S = st[r];
+    if (S.stop_iter != 0) return;  // uniform over the block
+    double tot[1];
+    sum_partials<1>(partials + (int64_t)r * nparts, nparts, tot);
     if (threadIdx.x != 0) return;
-    for (int r = 0; r < NR; r++) {
-        PcgState &S = st[r];
-        if (S.stop_iter != 0) continue;
-        S.hah = tot[r];
-        if (fabs(tot[r]) < 1e-14) {
-            S.stop_iter = it;
-            S.exit_flag = 2;  // E3X_PCG_HAH_BREAK
-            S.run_dir = 0;
-        } else {
-            S.lambda = S.gg / tot[r];
-        }
+    S.hah = tot[0];
+    if (fabs(tot[0]) < 1e-14) {
+        S.stop_iter = it;
+        S.exit_flag = 2;  // E3X_PCG_HAH_BREAK
+        S.run_dir = 0;
+    } else {
+        S.lambda = S.gg / tot[0];
     }
 }
 
@@ -118,7 +113,9 @@ __global__ __launch_bounds__(kFinBlock) void k_hah_finish(const double *partials
 // partials of z.g (f90:1919-1946).  DO_U: u -= lambda h here (iterations whose true
 // gradient needs the current u); otherwise the next iteration's apply moves u.
 template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
+__device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__restrict__ u, T *__restrict__ gv,
+                                                const T *__restrict__ hv, T *__restrict__ zv,
+                                                const T *__restrict__ ah, const T *__restrict__ minv)
 {
     bool act[NR];
     double lam[NR];
@@ -135,29 +132,37 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
 #pragma unroll
     for (int r = 0; r < NR; r++) part[r] = 0.0;
     for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
+        // the node's M^-1 block, once for all right-hand sides
+        double mi[6];
+        if constexpr (DO_Z) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) mi[q] = (double)minv[q * ns + m];
+        }
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             if (!act[r]) continue;
             const int64_t o = (int64_t)r * 3 * ns + m;
             if constexpr (DO_U) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) A.u[o + c * ns] = A.u[o + c * ns] - lam[r] * (double)A.h[o + c * ns];
+                for (int c = 0; c < 3; c++) u[o + c * ns] = u[o + c * ns] - lam[r] * (double)hv[o + c * ns];
             }
             double g[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                g[c] = (double)A.g[o + c * ns];
+                g[c] = (double)gv[o + c * ns];
                 if constexpr (UPDATE_G) {
-                    const T gn = (T)(g[c] - lam[r] * (double)A.ah[o + c * ns]);
-                    A.g[o + c * ns] = gn;
+                    const T gn = (T)(g[c] - lam[r] * (double)ah[o + c * ns]);
+                    gv[o + c * ns] = gn;
                     g[c] = (double)gn;
                 }
             }
             if constexpr (DO_Z) {
-                double z0, z1, z2;
-                precond_apply(A.minv, ns, m, g[0], g[1], g[2], z0, z1, z2);
+                // planes 00 11 22 01 02 12 (f90:1527-1533)
+                const double z0 = mi[0] * g[0] + mi[3] * g[1] + mi[4] * g[2];
+                const double z1 = mi[3] * g[0] + mi[1] * g[1] + mi[5] * g[2];
+                const double z2 = mi[4] * g[0] + mi[5] * g[1] + mi[2] * g[2];
                 const T t0 = (T)z0, t1 = (T)z1, t2 = (T)z2;
-                A.z[o] = t0; A.z[o + ns] = t1; A.z[o + 2 * ns] = t2;
+                zv[o] = t0; zv[o + ns] = t1; zv[o + 2 * ns] = t2;
                 part[r] += (double)t0 * g[0] + (double)t1 * g[1] + (double)t2 * g[2];
             }
         }
@@ -165,46 +170,47 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
     if constexpr (DO_Z) write_partials<NR>(part, A.partials);
 }
 
-// gg, rel_res, history, divergence reset or gamma, stop test (f90:1938-2007)
-template <int NR>
+template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
+{
+    pcg_update_body<T, NR, UPDATE_G, DO_U, DO_Z>(A, A.u, A.g, A.h, A.z, A.ah, A.minv);
+}
+
+// gg, rel_res, history, divergence reset or gamma, stop test (f90:1938-2007); block r
+// serves right-hand side r
 __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *partials, int nparts, PcgState *st,
                                                           double *hist, int it)
 {
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; r++) any |= st[r].stop_iter == 0;
-    if (!any) return;
-    double tot[NR];
-    sum_partials<NR>(partials, nparts, tot);
+    const int r = blockIdx.x;
+    PcgState &S = st[r];
+    if (S.stop_iter != 0) return;  // uniform over the block
+    double tot[1];
+    sum_partials<1>(partials + (int64_t)r * nparts, nparts, tot);
     if (threadIdx.x != 0) return;
-    for (int r = 0; r < NR; r++) {
-        PcgState &S = st[r];
-        if (S.stop_iter != 0) continue;
-        const double gglast = S.gg;
-        S.gg = tot[r];
-        const double rel = sqrt(S.gg / S.gg0);
-        S.rel_res = rel;
-        hist[(int64_t)r * S.ldemb + it - 1] = rel;
-        if (rel > 10.0 * S.rel_res_prev && it > 1) {
-            S.reset = 1;
-            S.rel_res_min = rel;
-            S.reset_count += 1;
-            S.gamma = 0.0;
-        } else {
-            S.reset = 0;
-            S.gamma = S.gg / gglast;
-            if (rel < S.rel_res_min) S.rel_res_min = rel;
-        }
-        S.rel_res_prev = rel;
-        if (rel < S.tol) {
-            S.stop_iter = it;
-            S.exit_flag = 1;  // E3X_PCG_CONVERGED
-            S.run_dir = 1;
-        } else if (it >= S.ldemb) {
-            S.stop_iter = it;
-            S.exit_flag = 3;  // E3X_PCG_MAXITER
-            S.run_dir = 1;
-        }
+    const double gglast = S.gg;
+    S.gg = tot[0];
+    const double rel = sqrt(S.gg / S.gg0);
+    S.rel_res = rel;
+    hist[(int64_t)r * S.ldemb + it - 1] = rel;
+    if (rel > 10.0 * S.rel_res_prev && it > 1) {
+        S.reset = 1;
+        S.rel_res_min = rel;
+        S.reset_count += 1;
+        S.gamma = 0.0;
+    } else {
+        S.reset = 0;
+        S.gamma = S.gg / gglast;
+        if (rel < S.rel_res_min) S.rel_res_min = rel;
+    }
+    S.rel_res_prev = rel;
+    if (rel < S.tol) {
+        S.stop_iter = it;
+        S.exit_flag = 1;  // E3X_PCG_CONVERGED
+        S.run_dir = 1;
+    } else if (it >= S.ldemb) {
+        S.stop_iter = it;
+        S.exit_flag = 3;  // E3X_PCG_MAXITER
+        S.run_dir = 1;
     }
 }
 
@@ -286,7 +292,7 @@ template <typename T, int NR>
 void iter_nr(Solver &s, int it, bool replace, bool prev_replace)
 {
     const int g = (int)vector_grid(s);
-    const dim3 vg(g), blk(kBlock), one(1);
+    const dim3 vg(g), blk(kBlock);
     // 1. direction and u step of the previous iteration, Ah = A h, the curvature (f90:1578-1907)
     (void)launch_apply_hah(s, sizeof(T) == 4, it, !prev_replace);
     PcgArgs<T> A = pcg_args<T>(s, it);
@@ -302,7 +308,7 @@ void iter_nr(Solver &s, int it, bool replace, bool prev_replace)
         (void)launch_resid(s);
         hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true>), vg, blk, 0, s.stream, A);
     }
-    hipLaunchKernelGGL(k_update_finish<NR>, one, dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.st, s.hist, it);
+    hipLaunchKernelGGL(k_update_finish, dim3(NR), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.st, s.hist, it);
 }
 
 template <typename T, int NR>
@@ -359,14 +365,8 @@ void finalize_t(Solver &s, int replace_every)
 
 hipError_t launch_hah_finish(Solver &s, int nparts, int it)
 {
-    switch (s.nrhs) {
-    case 1: hipLaunchKernelGGL(k_hah_finish<1>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 2: hipLaunchKernelGGL(k_hah_finish<2>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 3: hipLaunchKernelGGL(k_hah_finish<3>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 4: hipLaunchKernelGGL(k_hah_finish<4>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    case 5: hipLaunchKernelGGL(k_hah_finish<5>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    default: hipLaunchKernelGGL(k_hah_finish<6>, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, nparts, s.st, it); break;
-    }
+    hipLaunchKernelGGL(k_hah_finish, dim3((unsigned)s.nrhs), dim3(kFinBlock), 0, s.stream, (const double *)s.partials,
+                       nparts, s.st, it);
     return hipGetLastError();
 }
 

@@ -16,8 +16,8 @@
 // The next phase is the one of the first lane still pending (ballot + readlane).  Each
 // lane is served exactly once per octant (or once in all, when homogeneous), so a
 // node's sum runs over its octants in fixed order whatever its neighbours hold: results
-// are bitwise reproducible.  Right-hand sides are staged one at a time, which keeps the
-// LDS of a block at 17 KB (FP64) for any number of them.
+// are bitwise reproducible.  Each block serves one right-hand side, so the
+// kernel is the same for any number of them (17 KB of LDS in FP64).
 #include "e3x_device.hpp"
 
 #include <cmath>
@@ -27,12 +27,12 @@ namespace {
 
 template <typename T>
 struct StencilArgs {
-    int nx, ny, nz, ntx, nty;
+    int nx, ny, nz, ntx, nty, nrhs;
     int64_t ns;
     const uint16_t *pix;
     const T *dk, *sten;
-    const T *x;            // [NR][3][ns]
-    T *y;                  // [NR][3][ns] (kApply, kEnergy, kPlain)
+    const T *x;            // [nrhs][3][ns]
+    T *y;                  // [nrhs][3][ns] (kApply, kEnergy, kPlain)
     const double *b;       // kEnergy, kResid
     float *y32;            // kResid output
     double *partials;
@@ -47,13 +47,26 @@ struct StencilArgs {
 
 __device__ __forceinline__ int pmod(int v, int n) { v %= n; return v < 0 ? v + n : v; }
 
+// The phase tables are read-only while a stencil kernel runs.  Reading them through the
+// constant address space makes every wave-uniform coefficient load a scalar load
+// whatever the kernel stores elsewhere (through the global address space the compiler
+// must prove that no store clobbers them, which conditional stores defeat).
+template <typename T>
+using ctab = const __attribute__((address_space(4))) T;
+template <typename T>
+__device__ __forceinline__ ctab<T> *as_table(const T *p) { return (ctab<T> *)p; }
+
 // Component c of neighbour nb = (dz+1)*9 + (dy+1)*3 + (dx+1) relative to the LDS centre hr.
 #define E3X_NB(hr, c, nb) (hr)[(c) * HN + ((nb) / 9 - 1) * (HX * HY) + ((nb) / 3 % 3 - 1) * HX + ((nb) % 3 - 1)]
 
-// The body takes the arrays as restrict parameters: the output stores of one right-hand
-// side then provably leave the tables untouched, and the coefficient loads of the next
-// one stay scalar.
-template <typename T, int NR, int MODE>
+// The body takes the arrays as restrict parameters: the output stores then provably
+// leave the tables untouched, and coefficient loads after a store stay scalar.
+//
+// RB right-hand sides per block (grid: tiles x ceil(nrhs / RB)): the block stages the RB
+// halos, classifies its nodes once and applies every scalar coefficient it loads to all
+// RB of them.  A right-hand side past nrhs, or one that has stopped, is computed from
+// whatever its LDS slots hold and not stored.
+template <typename T, int MODE, int RB>
 __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint16_t *__restrict__ pix,
                                              const T *__restrict__ dk, const T *__restrict__ sten,
                                              const T *__restrict__ x, T *__restrict__ y,
@@ -62,13 +75,20 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
                                              T *__restrict__ hnew, double *__restrict__ u)
 {
     constexpr bool FUSED = MODE == kApplyFused;
-    __shared__ T hs[3 * HN];
+    constexpr bool HAS_ST = MODE == kApply || MODE == kApplyFused || MODE == kResid;
+    constexpr int NP = MODE == kEnergy ? 2 : 1;  // partial sums per right-hand side
+    __shared__ T hs[RB * 3 * HN];
     __shared__ int ps[VN];
     __shared__ int cnt[kBlock / 64];
     __shared__ short order[kBlock];
     __shared__ unsigned arrived;  // waves done, for the block partials
 
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    // grid (tiles, groups of RB right-hand sides); the XCD remap runs over the linear
+    // block id, whose remainder mod 8 is the XCD, and (tile, group) come from the
+    // remapped id
+    const int ntile = gridDim.x, ngroup = gridDim.y, nrhs = A.nrhs;
+    const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * ngroup);
+    const int grp = lin / ntile, bid = lin % ntile;
     const int tx = bid % A.ntx, ty = (bid / A.ntx) % A.nty, tz = bid / (A.ntx * A.nty);
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
@@ -78,21 +98,23 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // Interior tiles index without wrapping; boundary tiles wrap each coordinate.
     const bool interior = ox >= 1 && oy >= 1 && oz >= 1 && ox + TX + 1 <= nx && oy + TY + 1 <= ny && oz + TZ + 1 <= nz;
     const int corner = (oz - 1) * nxy + (oy - 1) * nx + (ox - 1);
-    unsigned live = (1u << NR) - 1u;  // right-hand sides to compute
-    if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kResid) {
-        live = 0u;
+
+    unsigned live = 0u;  // bit rb: right-hand side grp*RB + rb exists and still runs
 #pragma unroll
-        for (int r = 0; r < NR; r++)
-            if (A.st[r].stop_iter == 0) live |= 1u << r;
+    for (int rb = 0; rb < RB; rb++) {
+        const int r = grp * RB + rb;
+        if (r < nrhs && (!HAS_ST || A.st[r].stop_iter == 0)) live |= 1u << rb;
     }
-    auto stage = [&](int r) {
+    auto stage = [&](int rb) {
+        const int r = grp * RB + rb;
         const int64_t o = (int64_t)(3 * r) * ns;
+        T *hsr = hs + rb * 3 * HN;
         if constexpr (FUSED) {
             // this iteration's direction, h = z + gamma h_prev or h = z on a reset
             // (f90:1961-1996); the block's own nodes also store it and take the previous
             // step into u, u -= lambda h_prev (f90:1914-1918)
             const double gam = A.st[r].gamma, lam = A.st[r].lambda;
-            const bool rst = A.st[r].reset != 0, run = (live >> r) & 1u, dou = run && A.fuse_u;
+            const bool run = (live >> rb) & 1u, rst = A.st[r].reset != 0, dou = A.fuse_u != 0;
             auto put = [&](int q, int g) {
                 const int hx = q % HX, hy = (q / HX) % HY, hz = q / (HX * HY);
                 const bool own = run && hx >= 1 && hx <= TX && hy >= 1 && hy <= TY && hz >= 1 && hz <= TZ &&
@@ -102,7 +124,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
                     const int64_t i = o + c * ns + g;
                     const T zc = zv[i], hc = hold[i];
                     const T hv = rst ? zc : (T)((double)zc + gam * (double)hc);
-                    hs[c * HN + q] = hv;
+                    hsr[c * HN + q] = hv;
                     if (own) {
                         hnew[i] = hv;
                         if (dou) u[i] = u[i] - lam * (double)hc;
@@ -128,20 +150,20 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
                     const int q = tid + it * kBlock;
                     if (q < HN) {
                         const int g = corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX;
-                        hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+                        hsr[q] = x0[g]; hsr[HN + q] = x1[g]; hsr[2 * HN + q] = x2[g];
                     }
                 }
             } else {
                 for (int q = tid; q < HN; q += kBlock) {
                     const int g = pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
                                   nxy * pmod(oz + q / (HX * HY) - 1, nz);
-                    hs[q] = x0[g]; hs[HN + q] = x1[g]; hs[2 * HN + q] = x2[g];
+                    hsr[q] = x0[g]; hsr[HN + q] = x1[g]; hsr[2 * HN + q] = x2[g];
                 }
             }
         }
     };
 
-    stage(0);  // issued before the early exit below, so its latency overlaps the state loads
+    // the voxel phases first: their loads overlap the state loads of the early exit
     if (interior) {
 #pragma unroll
         for (int it = 0; it < (VN + kBlock - 1) / kBlock; it++) {
@@ -153,7 +175,12 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             ps[q] = pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
                         nxy * pmod(oz + q / (VX * VY) - 1, nz)];
     }
+    // One staging site only: with a second one (staging before this exit) the compiler
+    // no longer proves the tables unclobbered and loads the coefficients per lane.
     if (!live) return;  // uniform over the block
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+        if ((live >> rb) & 1u) stage(rb);
     __syncthreads();
 
     // Stable compaction: homogeneous nodes first, then mixed ones.
@@ -189,134 +216,163 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 #pragma unroll
     for (int a = 1; a < 8; a++) homog &= pv[a] == pv[0];
     const bool hlane = active && homog, mlane = active && !homog;
-    const T *hr = hs + (lz + 1) * (HX * HY) + (ly + 1) * HX + (lx + 1);
+    const T *hr = hs + (lz + 1) * (HX * HY) + (ly + 1) * HX + (lx + 1);  // rhs rb: hr + rb * 3 * HN
 
-    double part[MODE == kEnergy ? 2 * NR : NR];
+    T acc[RB][3];
 #pragma unroll
-    for (int q = 0; q < (MODE == kEnergy ? 2 * NR : NR); q++) part[q] = 0.0;
-
+    for (int rb = 0; rb < RB; rb++) acc[rb][0] = acc[rb][1] = acc[rb][2] = T(0);
+    if (__ballot(hlane)) {
+        bool pend = hlane;
+        while (true) {
+            const unsigned long long pm = __ballot(pend);
+            if (pm == 0ull) break;
+            const int P = __builtin_amdgcn_readlane(pv[0], __builtin_ctzll(pm));
+            ctab<T> *S = as_table(sten) + (int64_t)P * kStenHalf;
+            T tt[RB][3];
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        if (!((live >> r) & 1u)) continue;
-        if (r != 0) {
-            __syncthreads();  // every wave is done with the previous right-hand side
-            stage(r);
-            __syncthreads();
+            for (int rb = 0; rb < RB; rb++) tt[rb][0] = tt[rb][1] = tt[rb][2] = T(0);
+#pragma unroll
+            for (int k = 0; k < 13; k++) {
+#pragma unroll
+                for (int n = 0; n < 3; n++) {  // neighbour +delta
+                    const T s0 = S[k * 9 + 0 * 3 + n], s1 = S[k * 9 + 1 * 3 + n], s2 = S[k * 9 + 2 * 3 + n];
+#pragma unroll
+                    for (int rb = 0; rb < RB; rb++) {
+                        const T hp = E3X_NB(hr + rb * 3 * HN, n, 14 + k);
+                        tt[rb][0] += s0 * hp; tt[rb][1] += s1 * hp; tt[rb][2] += s2 * hp;
+                    }
+                }
+#pragma unroll
+                for (int n = 0; n < 3; n++) {  // neighbour -delta: S[-d][j][n] = S[d][n][j]
+                    const T s0 = S[k * 9 + n * 3 + 0], s1 = S[k * 9 + n * 3 + 1], s2 = S[k * 9 + n * 3 + 2];
+#pragma unroll
+                    for (int rb = 0; rb < RB; rb++) {
+                        const T hq = E3X_NB(hr + rb * 3 * HN, n, 12 - k);
+                        tt[rb][0] += s0 * hq; tt[rb][1] += s1 * hq; tt[rb][2] += s2 * hq;
+                    }
+                }
+            }
+            ctab<T> *Cc = S + 117;  // centre: 00 11 22 01 02 12
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                const T *h0 = hr + rb * 3 * HN;
+                const T c0 = h0[0], c1 = h0[HN], c2 = h0[2 * HN];
+                tt[rb][0] += Cc[0] * c0; tt[rb][0] += Cc[3] * c1; tt[rb][0] += Cc[4] * c2;
+                tt[rb][1] += Cc[3] * c0; tt[rb][1] += Cc[1] * c1; tt[rb][1] += Cc[5] * c2;
+                tt[rb][2] += Cc[4] * c0; tt[rb][2] += Cc[5] * c1; tt[rb][2] += Cc[2] * c2;
+            }
+            if (pend && pv[0] == P) {
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++) {
+                    acc[rb][0] = tt[rb][0]; acc[rb][1] = tt[rb][1]; acc[rb][2] = tt[rb][2];
+                }
+                pend = false;
+            }
         }
-        T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-        if (__ballot(hlane)) {
-            bool pend = hlane;
+    }
+    if (__ballot(mlane)) {
+#pragma unroll 1
+        for (int a = 0; a < 8; a++) {
+            // voxel a of the node spans nodes (PX(b)-PX(a), PY(b)-PY(a), PZ(b)-PZ(a)), b = 0..7
+            const T *hb = hr - (PZ(a) * (HX * HY) + PY(a) * HX + PX(a));
+            T hv[RB][24];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                for (int b = 0; b < 8; b++)
+#pragma unroll
+                    for (int n = 0; n < 3; n++)
+                        hv[rb][b * 3 + n] = hb[rb * 3 * HN + n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
+            const int pa = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
+            bool pend = mlane;
             while (true) {
                 const unsigned long long pm = __ballot(pend);
                 if (pm == 0ull) break;
-                const int P = __builtin_amdgcn_readlane(pv[0], __builtin_ctzll(pm));
-                const T *S = sten + (int64_t)P * kStenHalf;
-                T t0 = T(0), t1 = T(0), t2 = T(0);
+                const int P = __builtin_amdgcn_readlane(pa, __builtin_ctzll(pm));
+                ctab<T> *D = as_table(dk) + (int64_t)P * 576 + a * 72;
+                T tt[RB][3];
 #pragma unroll
-                for (int k = 0; k < 13; k++) {
+                for (int rb = 0; rb < RB; rb++) tt[rb][0] = tt[rb][1] = tt[rb][2] = T(0);
 #pragma unroll
-                    for (int n = 0; n < 3; n++) {  // neighbour +delta
-                        const T hp = E3X_NB(hr, n, 14 + k);
-                        t0 += S[k * 9 + 0 * 3 + n] * hp;
-                        t1 += S[k * 9 + 1 * 3 + n] * hp;
-                        t2 += S[k * 9 + 2 * 3 + n] * hp;
-                    }
+                for (int j = 0; j < 24; j++) {
+                    const T d0 = D[j], d1 = D[24 + j], d2 = D[48 + j];
 #pragma unroll
-                    for (int n = 0; n < 3; n++) {  // neighbour -delta: S[-d][j][n] = S[d][n][j]
-                        const T hq = E3X_NB(hr, n, 12 - k);
-                        t0 += S[k * 9 + n * 3 + 0] * hq;
-                        t1 += S[k * 9 + n * 3 + 1] * hq;
-                        t2 += S[k * 9 + n * 3 + 2] * hq;
+                    for (int rb = 0; rb < RB; rb++) {
+                        tt[rb][0] += d0 * hv[rb][j]; tt[rb][1] += d1 * hv[rb][j]; tt[rb][2] += d2 * hv[rb][j];
                     }
                 }
-                const T *Cc = S + 117;  // centre: 00 11 22 01 02 12
-                const T c0 = hr[0], c1 = hr[HN], c2 = hr[2 * HN];
-                t0 += Cc[0] * c0; t0 += Cc[3] * c1; t0 += Cc[4] * c2;
-                t1 += Cc[3] * c0; t1 += Cc[1] * c1; t1 += Cc[5] * c2;
-                t2 += Cc[4] * c0; t2 += Cc[5] * c1; t2 += Cc[2] * c2;
-                if (pend && pv[0] == P) {
-                    acc0 = t0; acc1 = t1; acc2 = t2;
+                if (pend && pa == P) {
+#pragma unroll
+                    for (int rb = 0; rb < RB; rb++) {
+                        acc[rb][0] += tt[rb][0]; acc[rb][1] += tt[rb][1]; acc[rb][2] += tt[rb][2];
+                    }
                     pend = false;
                 }
             }
         }
-        if (__ballot(mlane)) {
-#pragma unroll 1
-            for (int a = 0; a < 8; a++) {
-                // voxel a of the node spans nodes (PX(b)-PX(a), PY(b)-PY(a), PZ(b)-PZ(a)), b = 0..7
-                const T *hb = hr - (PZ(a) * (HX * HY) + PY(a) * HX + PX(a));
-                T hv[24];
-#pragma unroll
-                for (int b = 0; b < 8; b++)
-#pragma unroll
-                    for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
-                const int pa = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
-                bool pend = mlane;
-                while (true) {
-                    const unsigned long long pm = __ballot(pend);
-                    if (pm == 0ull) break;
-                    const int P = __builtin_amdgcn_readlane(pa, __builtin_ctzll(pm));
-                    const T *D = dk + (int64_t)P * 576 + a * 72;
-                    T t0 = T(0), t1 = T(0), t2 = T(0);
-#pragma unroll
-                    for (int j = 0; j < 24; j++) {
-                        t0 += D[j] * hv[j];
-                        t1 += D[24 + j] * hv[j];
-                        t2 += D[48 + j] * hv[j];
-                    }
-                    if (pend && pa == P) {
-                        acc0 += t0; acc1 += t1; acc2 += t2;
-                        pend = false;
-                    }
-                }
-            }
-        }
+    }
 
-        if (active) {
-            const T acc[3] = {acc0, acc1, acc2};
-            if constexpr (MODE == kApply || MODE == kApplyFused) {
-                double pr = 0.0;
+    double part[RB * NP];
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    y[(int64_t)(3 * r + j) * ns + m] = acc[j];
-                    pr += (double)hr[j * HN] * (double)acc[j];
-                }
-                part[r] += pr;
-            } else if constexpr (MODE == kEnergy) {
-                double e = 0.0, gsq = 0.0;
+    for (int q = 0; q < RB * NP; q++) part[q] = 0.0;
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const int64_t q = (int64_t)(3 * r + j) * ns + m;
-                    const double uq = (double)hr[j * HN], bq = bvec[q], kx = (double)acc[j];
-                    e = e + 0.5 * uq * kx + bq * uq;  // f90:1336
-                    const double gq = kx + bq;         // f90:1337
-                    y[q] = (T)gq;
-                    gsq += gq * gq;                     // f90:452
-                }
-                part[r] += e;
-                part[NR + r] += gsq;
-            } else if constexpr (MODE == kResid) {
+    for (int rb = 0; rb < RB; rb++) {
+        if (!active || !((live >> rb) & 1u)) continue;
+        const int64_t ro = (int64_t)(3 * (grp * RB + rb)) * ns;  // this right-hand side's vectors
+        const T *h0 = hr + rb * 3 * HN;
+        if constexpr (MODE == kApply || MODE == kApplyFused) {
+            double pr = 0.0;
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const int64_t q = (int64_t)(3 * r + j) * ns + m;
-                    y32[q] = (float)((double)acc[j] + bvec[q]);
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 3; j++) y[(int64_t)(3 * r + j) * ns + m] = acc[j];
+            for (int j = 0; j < 3; j++) {
+                y[ro + j * ns + m] = acc[rb][j];
+                pr += (double)h0[j * HN] * (double)acc[rb][j];
             }
+            part[rb] = pr;
+        } else if constexpr (MODE == kEnergy) {
+            double e = 0.0, gsq = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int64_t q = ro + j * ns + m;
+                const double uq = (double)h0[j * HN], bq = bvec[q], kx = (double)acc[rb][j];
+                e = e + 0.5 * uq * kx + bq * uq;  // f90:1336
+                const double gq = kx + bq;         // f90:1337
+                y[q] = (T)gq;
+                gsq += gq * gq;                     // f90:452
+            }
+            part[rb] = e;
+            part[RB + rb] = gsq;
+        } else if constexpr (MODE == kResid) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int64_t q = ro + j * ns + m;
+                y32[q] = (float)((double)acc[rb][j] + bvec[q]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 3; j++) y[ro + j * ns + m] = acc[rb][j];
         }
     }
-    if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kEnergy) write_partials_nowait(part, A.partials, &arrived);
+    // partial slots [q * nrhs + r][tile]: h.Ah (or utot), then gg for kEnergy; none for a
+    // right-hand side past nrhs
+    if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kEnergy) {
+        int64_t slot[RB * NP];
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const int r = grp * RB + rb;
+            slot[rb] = r < nrhs ? (int64_t)r * ntile + bid : -1;
+            if constexpr (MODE == kEnergy) slot[RB + rb] = r < nrhs ? (int64_t)(nrhs + r) * ntile + bid : -1;
+        }
+        write_partials_nowait(part, A.partials, slot, &arrived);
+    }
 }
 #undef E3X_NB
 
-// Six waves per SIMD: enough to cover the scalar-cache latency of the coefficients,
-// within the 80 VGPRs the body needs.
-template <typename T, int NR, int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_stencil(StencilArgs<T> A)
+// At least WPE waves per SIMD (six: the 80 VGPRs of the FP64 body, enough to cover the
+// scalar-cache latency of the coefficients; eight for the FP32 bodies, which fit in 64).
+template <typename T, int MODE, int RB, int WPE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_stencil(StencilArgs<T> A)
 {
-    stencil_body<T, NR, MODE>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold, A.hnew, A.u);
+    stencil_body<T, MODE, RB>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold, A.hnew, A.u);
 }
 
 template <int NR>
@@ -336,25 +392,23 @@ template <typename T>
 StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T *x, T *y)
 {
     StencilArgs<T> A{};
-    A.nx = s.nx; A.ny = s.ny; A.nz = s.nz;
+    A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.nrhs = s.nrhs;
     A.ntx = (s.nx + TX - 1) / TX; A.nty = (s.ny + TY - 1) / TY;
     A.ns = s.ns; A.pix = s.pix; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
     A.partials = s.partials; A.st = s.st;
     return A;
 }
 
+// One right-hand side per block (grid y = nrhs).  The body also serves RB > 1 (shared
+// staging of the phases, classification and coefficient loads), but at 200^3 two FP32
+// right-hand sides per block ran 2-4 % slower than one: the per-rhs FMA, LDS and
+// staging work dominates, and the second halo costs occupancy.
 template <typename T, int MODE>
 void dispatch_stencil(const Solver &s, const StencilArgs<T> &A)
 {
-    const dim3 grid((unsigned)stencil_blocks(s)), block(kBlock);
-    switch (s.nrhs) {
-    case 1: hipLaunchKernelGGL((k_stencil<T, 1, MODE>), grid, block, 0, s.stream, A); break;
-    case 2: hipLaunchKernelGGL((k_stencil<T, 2, MODE>), grid, block, 0, s.stream, A); break;
-    case 3: hipLaunchKernelGGL((k_stencil<T, 3, MODE>), grid, block, 0, s.stream, A); break;
-    case 4: hipLaunchKernelGGL((k_stencil<T, 4, MODE>), grid, block, 0, s.stream, A); break;
-    case 5: hipLaunchKernelGGL((k_stencil<T, 5, MODE>), grid, block, 0, s.stream, A); break;
-    default: hipLaunchKernelGGL((k_stencil<T, 6, MODE>), grid, block, 0, s.stream, A); break;
-    }
+    const dim3 grid((unsigned)stencil_blocks(s), (unsigned)s.nrhs), block(kBlock);
+    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_stencil<T, MODE, 1, 8>), grid, block, 0, s.stream, A);
+    else hipLaunchKernelGGL((k_stencil<T, MODE, 1, 6>), grid, block, 0, s.stream, A);
 }
 
 }  // namespace

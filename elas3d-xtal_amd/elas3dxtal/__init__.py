@@ -4,7 +4,8 @@ The compute path is libe3x.so (HIP kernels behind the C ABI of include/e3x.h);
 this package only binds it.
 """
 from . import _lib
-from .solver import AML_DEFAULT, LDEMB_DEFAULT, TOL_DEFAULT, Elas3D, SolveResult, solve, unit_strain
+from .solver import (AML_DEFAULT, LDEMB_DEFAULT, TOL_DEFAULT, Elas3D, SolveResult, effective_stiffness, solve,
+                     unit_strain)
 
-__all__ = ["Elas3D", "SolveResult", "solve", "unit_strain", "TOL_DEFAULT", "AML_DEFAULT",
+__all__ = ["Elas3D", "SolveResult", "solve", "effective_stiffness", "unit_strain", "TOL_DEFAULT", "AML_DEFAULT",
            "LDEMB_DEFAULT", "_lib"]

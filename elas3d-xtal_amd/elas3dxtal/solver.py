@@ -134,6 +134,22 @@ class Elas3D:
         L.check(self._lib.e3x_stress(self._h, L.dptr(s), L.dptr(e)))
         return s.reshape(self.nrhs, 6), e.reshape(self.nrhs, 6)
 
+    def effective_stiffness(self, c11, c12, c44, magnitude=AML_DEFAULT, phasemod=(0.0, 0.0), flag_m=1,
+                            precision=L.PREC_FP64, replace_every=50, tol=TOL_DEFAULT, ldemb=LDEMB_DEFAULT):
+        """6x6 effective stiffness from the six unit load cases solved as six right-hand
+        sides (e3x_effective_stiffness; the handle needs nrhs = 6).  Returns (C_eff,
+        lstep[6], exit[6]); Voigt order xx yy zz xz yz xy, engineering shear strains."""
+        pm = np.zeros(4)
+        pmv = np.asarray(phasemod, float).ravel()
+        pm[:pmv.size] = pmv
+        ce = np.zeros(36)
+        lstep = np.zeros(6, np.int32)
+        ex = np.zeros(6, np.int32)
+        L.check(self._lib.e3x_effective_stiffness(self._h, c11, c12, c44, L.dptr(pm), flag_m, magnitude, precision,
+                                                  replace_every if precision else 0, C.c_float(tol), ldemb,
+                                                  L.dptr(ce), L.iptr(lstep), L.iptr(ex)))
+        return ce.reshape(6, 6), lstep, ex
+
     def stress_fullfield(self, rhs=0):
         """stress_fullfield_OpenMP (f90:2639-2833) -> stress (6, ns), vm (ns)."""
         sf, vm = np.zeros(6 * self.ns), np.zeros(self.ns)
@@ -215,3 +231,14 @@ def solve(pix, orientation, shape, nphase, c11, c12, c44, strain6, phasemod=(0.0
         return res, s
     s.close()
     return res
+
+
+def effective_stiffness(pix, orientation, shape, nphase, c11, c12, c44, magnitude=AML_DEFAULT,
+                        phasemod=(0.0, 0.0), flag_m=1, precision=L.PREC_FP64, replace_every=50,
+                        tol=TOL_DEFAULT, ldemb=LDEMB_DEFAULT, device=0):
+    """Effective 6x6 stiffness of one microstructure: the six unit macro strains as six
+    right-hand sides of one batched PCG.  Returns (C_eff, lstep[6], exit[6])."""
+    with Elas3D(shape, nphase, nrhs=6, device=device) as s:
+        s.ppixel(pix, orientation)
+        return s.effective_stiffness(c11, c12, c44, magnitude=magnitude, phasemod=phasemod, flag_m=flag_m,
+                                     precision=precision, replace_every=replace_every, tol=tol, ldemb=ldemb)

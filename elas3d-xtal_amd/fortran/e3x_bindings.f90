@@ -98,6 +98,21 @@
           real(c_double) :: stress6(*), strain6(*)
         end function
 
+        ! six unit load cases as six right-hand sides -> 6x6 effective stiffness
+        ! (no counterpart in the reference; the handle needs nrhs = 6)
+        integer(c_int) function e3x_effective_stiffness(h, c11, c12, c44, phasemod, flag_m, magnitude, &
+            precision, replace_every, tol, ldemb, c_eff, lstep, exit_flag) &
+            bind(C, name="e3x_effective_stiffness")
+          import :: c_int, c_ptr, c_double, c_float
+          type(c_ptr), value :: h
+          real(c_double), value :: c11, c12, c44, magnitude
+          real(c_double), intent(in) :: phasemod(*)
+          integer(c_int), value :: flag_m, precision, replace_every, ldemb
+          real(c_float), value :: tol
+          real(c_double) :: c_eff(*)
+          integer(c_int) :: lstep(*), exit_flag(*)
+        end function
+
         ! stress_fullfield_OpenMP, computation (f90:2639-2833)
         integer(c_int) function e3x_stress_fullfield(h, rhs, stress, vm) &
             bind(C, name="e3x_stress_fullfield")

@@ -158,6 +158,21 @@ int e3x_stress(e3x_solver *s, double *stress6, double *strain6);
  */
 int e3x_stress_fullfield(e3x_solver *s, int rhs, double *stress, double *vm);
 
+/*
+ * Effective stiffness of the loaded microstructure (no counterpart in the reference,
+ * which solves one load case per run): on a handle created with nrhs = 6, runs the main
+ * program's sequence (f90:417-514: femat, affine guess, energy, dembx, stress) for the six
+ * unit macro strains of size `magnitude` (3 normal; 3 engineering shears xz, yz, xy,
+ * tensor component magnitude/2) as six right-hand sides of one batched PCG, and returns
+ * C_eff[36] (row-major, C_eff[6*i + j] = d<sigma_i>/d<eps_j>, Voigt order xx yy zz xz
+ * yz xy, engineering shear strains) from the averaged stresses and strains:
+ * C_eff = S E^-1.  precision / replace_every as in e3x_dembx_ex; lstep[6], exit_flag[6]
+ * may be NULL.
+ */
+int e3x_effective_stiffness(e3x_solver *s, double c11, double c12, double c44, const double *phasemod,
+                            int flag_m, double magnitude, int precision, int replace_every, float tol,
+                            int ldemb, double *C_eff, int *lstep, int *exit_flag);
+
 /* Copy a field (enum e3x_field) of right-hand side `rhs` to / from host double[3*ns]. */
 int e3x_get_field(e3x_solver *s, int which, int rhs, double *host);
 int e3x_set_field(e3x_solver *s, int which, int rhs, const double *host);

@@ -34,3 +34,19 @@ def voronoi_case(shape, ngrain, seed, pore_frac=0.02):
 # pore moduli (K, G) = (-160, 10) GPa make the stiffness indefinite, and the
 # preconditioned residual jumps by more than 10x several times before it converges.
 RESET_CASE = dict(shape=(6, 6, 6), ngrain=2, seed=2, pore_frac=0.1, phasemod=(-160e9, 10e9))
+
+
+def unit_strains(magnitude):
+    """Six unit load cases {exx, eyy, ezz, exz, eyz, exy} (tensor components; the shears
+    are engineering strains of `magnitude`, tensor component half of it)."""
+    e = np.zeros((6, 6))
+    for k in range(6):
+        e[k, k] = magnitude if k < 3 else 0.5 * magnitude
+    return e
+
+
+def ceff_from_cases(stress6, strain6):
+    """C_eff with <sigma>_k = C_eff <eps>_k for the six cases (rows k of the inputs)."""
+    S = np.asarray(stress6, float).T
+    E = np.asarray(strain6, float).T
+    return S @ np.linalg.inv(E)

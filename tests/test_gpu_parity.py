@@ -9,9 +9,9 @@ import pytest
 
 import _indep as ind
 import _oracle as oc
-from _cases import RESET_CASE, voronoi_case
+from _cases import RESET_CASE, ceff_from_cases, unit_strains, voronoi_case
 from elas3dxtal import _lib as L
-from elas3dxtal.solver import Elas3D, solve
+from elas3dxtal.solver import Elas3D, effective_stiffness, solve
 
 pytestmark = pytest.mark.gpu
 
@@ -245,16 +245,49 @@ def test_mixed_precision_against_fp64_oracle(replace_every):
     assert _rel(res.strain6[0], ro["strain6"]) < 1e-12
 
 
-def test_multi_rhs_equals_single_solves():
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_multi_rhs_equals_single_solves(precision):
+    """Three right-hand sides in one PCG give each one's single solve (the mixed form's
+    kernels serve two per block, so the third shares its block with an unused slot)."""
     shape = (12, 10, 8)
     pix, rod = voronoi_case(shape, 6, seed=500)
     strains = np.array([[oc.AML_F32, 0, 0, 0, 0, 0], [0, 0, 0, 0.5e-3, 0, 0], [0, 0.7e-3, 0, 0, 0, 0.2e-3]])
-    batch = solve(pix, rod, shape, 7, C11, C12, C44, strains)
+    batch = solve(pix, rod, shape, 7, C11, C12, C44, strains, precision=precision)
     for r in range(3):
-        one = solve(pix, rod, shape, 7, C11, C12, C44, strains[r])
+        one = solve(pix, rod, shape, 7, C11, C12, C44, strains[r], precision=precision)
         assert batch.lstep[r] == one.lstep[0]
         assert abs(batch.utot[r] - one.utot[0]) <= 1e-13 * abs(one.utot[0])
         assert _rel(batch.stress6[r], one.stress6[0]) < 1e-13
+
+
+@pytest.mark.parametrize("precision,tol", [(L.PREC_FP64, 1e-10), (L.PREC_MIXED, 1e-6)])
+def test_effective_stiffness_matches_oracle(precision, tol):
+    """Six unit load cases as six right-hand sides of one PCG: the 6x6 effective
+    stiffness against six oracle solves (FP64 within 1e-10, the mixed form within 1e-6)."""
+    shape = (10, 9, 8)
+    ng = 5
+    pix, rod = voronoi_case(shape, ng, seed=800, pore_frac=0.02)
+    S, E = [], []
+    for e6 in unit_strains(oc.AML_F32):
+        r = oc.solve(shape, ng + 1, pix, rod, C11, C12, C44, list(e6), want_fields=False)
+        S.append(r["stress6"])
+        E.append(r["strain6"])
+    ce_o = ceff_from_cases(S, E)
+    ce, lstep, ex = effective_stiffness(pix, rod, shape, ng + 1, C11, C12, C44, magnitude=oc.AML_F32,
+                                        precision=precision)
+    assert np.all(ex == L.PCG_CONVERGED)
+    assert _rel(ce, ce_o) < tol
+
+
+def test_effective_stiffness_single_crystal():
+    shape = (6, 5, 4)
+    ce, lstep, ex = effective_stiffness(np.ones(120, np.int32), np.zeros(3), shape, 2, C11, C12, C44)
+    c = np.zeros((6, 6))
+    c[:3, :3] = C12
+    c[[0, 1, 2], [0, 1, 2]] = C11
+    c[[3, 4, 5], [3, 4, 5]] = C44
+    assert np.all(ex == L.PCG_HAH_BREAK)      # the affine guess is the solution
+    assert _rel(ce, c) < 1e-12
 
 
 def test_bad_phase_label_and_call_order():

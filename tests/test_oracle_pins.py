@@ -352,3 +352,51 @@ def test_assig_fractions():
     pix = np.array([1, 1, 2, 3, 3, 3, 9], np.int32)
     prob = oc.assig(pix, 3)
     assert np.array_equal(prob, np.array([2, 1, 3]) / 7.0)
+
+
+# ---------------------------------------------------------------- effective stiffness
+def _oracle_ceff(shape, nphase, pix, rod, magnitude=1e-3):
+    from _cases import ceff_from_cases, unit_strains
+    S, E = [], []
+    for e6 in unit_strains(magnitude):
+        r = oc.solve(shape, nphase, pix, rod, C11, C12, C44, list(e6), want_fields=False)
+        S.append(r["stress6"])
+        E.append(r["strain6"])
+    return ceff_from_cases(S, E)
+
+
+def test_effective_stiffness_single_crystal_is_cubic():
+    """One grain without pores: the affine field is the solution, C_eff = the cubic matrix
+    (xz, yz, xy shear order)."""
+    shape = (4, 3, 5)
+    ns = int(np.prod(shape))
+    ce = _oracle_ceff(shape, 2, np.ones(ns, np.int32), np.zeros(3))
+    c = np.zeros((6, 6))
+    c[:3, :3] = C12
+    c[[0, 1, 2], [0, 1, 2]] = C11
+    c[[3, 4, 5], [3, 4, 5]] = C44
+    assert _rel(ce, c) < 1e-12
+
+
+def test_effective_stiffness_between_reuss_and_voigt():
+    """A pore-free polycrystal: C_eff is symmetric and lies between the Reuss and Voigt
+    averages of the grain stiffnesses (the affine field is admissible, and the discrete
+    solution is stiffer than the continuum one)."""
+    from _cases import voronoi_case
+    shape = (8, 7, 6)
+    ng = 4
+    pix, rod = voronoi_case(shape, ng, seed=700, pore_frac=0.0)
+    ce = _oracle_ceff(shape, ng + 1, pix, rod)
+    # averages use voxel-centre strains (f90:2149-2408), not the energy-consistent
+    # average, so C_eff is symmetric only to the discretisation (~4e-5 here)
+    assert _rel(ce, ce.T) < 2e-4
+    rot = oc.rotate_stiffness(rod, C11, C12, C44).reshape(ng, 6, 6)
+    frac = np.bincount(pix, minlength=ng + 1)[1:] / pix.size
+    voigt = np.einsum("g,gij->ij", frac, rot)
+    reuss = np.linalg.inv(np.einsum("g,gij->ij", frac, np.linalg.inv(rot)))
+    sym = 0.5 * (ce + ce.T)
+    scale = np.max(np.abs(voigt))
+    # cubic grains share one bulk modulus: along the hydrostatic strain Voigt is exact
+    assert np.min(np.linalg.eigvalsh(voigt - sym)) > -1e-6 * scale
+    assert np.min(np.linalg.eigvalsh(sym - reuss)) > -1e-6 * scale
+    assert np.max(np.linalg.eigvalsh(voigt - sym)) > 1e-3 * scale      # strictly inside
