@@ -62,6 +62,15 @@ int check_handle(const e3x_solver *h)
     return E3X_OK;
 }
 
+// A handle whose PCG is in flight (e3x_dembx_begin .. e3x_dembx_end) takes no other call.
+int check_idle(const e3x_solver *h)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (reinterpret_cast<const Solver *>(h)->run.on)
+        return fail(E3X_ESTATE, "a PCG is in progress on this handle (e3x_dembx_end first)");
+    return E3X_OK;
+}
+
 Solver &S(e3x_solver *h) { return *reinterpret_cast<Solver *>(h); }
 
 int ensure_mixed_buffers(Solver &s)
@@ -93,6 +102,25 @@ int ensure_hist(Solver &s, int ldemb)
 
 }  // namespace
 
+namespace e3x {
+
+int solver_set_nphase(Solver &s, int nphase)
+{
+    if (s.run.on) return fail(E3X_ESTATE, "a PCG is in progress on this handle");
+    if (nphase < 1 || nphase > s.nphase_cap)
+        return fail(E3X_EINVAL, "nphase exceeds the phases this handle was created for");
+    s.nphase = nphase;
+    s.have_pix = s.have_femat = s.have_precond = s.have_precond32 = false;
+    return E3X_OK;
+}
+
+int report_error(int code, const char *msg)
+{
+    return fail(code, msg ? msg : "");
+}
+
+}  // namespace e3x
+
 extern "C" {
 
 const char *e3x_last_error(void) { return g_err.c_str(); }
@@ -122,7 +150,8 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     if (device < 0 || device >= ndev) return fail(E3X_EINVAL, "device index out of range");
     Solver *s = new (std::nothrow) Solver();
     if (!s) return fail(E3X_ENOMEM, "host allocation failed");
-    s->nx = nx; s->ny = ny; s->nz = nz; s->nphase = nphase; s->nrhs = nrhs; s->device = device;
+    s->nx = nx; s->ny = ny; s->nz = nz; s->nphase = nphase; s->nphase_cap = nphase; s->nrhs = nrhs;
+    s->device = device;
     s->ns = (int64_t)nx * ny * nz;
     auto bail = [&](int code) {
         release(*s);
@@ -194,7 +223,7 @@ int e3x_memory_bytes(const e3x_solver *h, int64_t *bytes)
 
 int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orientation)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!pix) return fail(E3X_EINVAL, "pix is NULL");
     std::vector<uint16_t> p16((size_t)s.ns);
@@ -225,7 +254,7 @@ int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orie
 
 int e3x_assig(e3x_solver *h, double *prob)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!prob) return fail(E3X_EINVAL, "prob is NULL");
     if (!s.have_pix) return fail(E3X_ESTATE, "e3x_assig before e3x_set_microstructure");
@@ -237,7 +266,7 @@ int e3x_assig(e3x_solver *h, double *prob)
 int e3x_femat(e3x_solver *h, double c11, double c12, double c44, const double *phasemod, int flag_m,
               const double *strain6, double *C_out)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_pix) return fail(E3X_ESTATE, "e3x_femat before e3x_set_microstructure");
     if (flag_m != 0 && flag_m != 1) return fail(E3X_EINVAL, "flag_m must be 0 or 1");
@@ -271,7 +300,7 @@ int e3x_femat(e3x_solver *h, double c11, double c12, double c44, const double *p
 
 int e3x_affine_guess(e3x_solver *h)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_femat) return fail(E3X_ESTATE, "e3x_affine_guess before e3x_femat");
     E3X_HIP(hipSetDevice(s.device));
@@ -282,7 +311,7 @@ int e3x_affine_guess(e3x_solver *h)
 
 int e3x_energy(e3x_solver *h, double *utot, double *gg)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_femat) return fail(E3X_ESTATE, "e3x_energy before e3x_femat");
     E3X_HIP(hipSetDevice(s.device));
@@ -296,12 +325,13 @@ int e3x_energy(e3x_solver *h, double *utot, double *gg)
     return E3X_OK;
 }
 
-int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *gg0, float tol, int ldemb,
-                 int kkk, int *lstep, double *gg, double *history, int *exit_flag, int *reset_count)
+int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const double *gg0, float tol, int ldemb,
+                    int kkk)
 {
     if (int rc = check_handle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_femat) return fail(E3X_ESTATE, "e3x_dembx before e3x_femat");
+    if (s.run.on) return fail(E3X_ESTATE, "a PCG is already in progress on this handle");
     if (!gg0) return fail(E3X_EINVAL, "gg0 is NULL");
     if (precision != E3X_PREC_FP64 && precision != E3X_PREC_MIXED)
         return fail(E3X_EINVAL, "precision must be E3X_PREC_FP64 or E3X_PREC_MIXED");
@@ -328,42 +358,95 @@ int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *
         if (kkk != 1) E3X_HIP(launch_to_f32(s.h, s.h32, nv, s.stream));
     }
     E3X_HIP(launch_pcg_init(s, mixed, kkk, ldemb, tol, gg0));
+    Solver::PcgRun &R = s.run;
+    R = Solver::PcgRun{};
+    R.on = true;
+    R.mixed = mixed;
+    R.replace_every = mixed ? replace_every : 0;
+    R.ldemb = ldemb;
+    R.chunk = s.ns >= (1 << 21) ? 4 : (s.ns >= (1 << 17) ? 16 : 32);
+    return E3X_OK;
+}
 
-    // PCG loop: every kernel reads the per-rhs stop state on the device, so the host
-    // only polls (double-buffered, one chunk behind) to stop launching.
-    const int chunk = s.ns >= (1 << 21) ? 4 : (s.ns >= (1 << 17) ? 16 : 32);
-    int it = 1, k = 0;
+// Every kernel of the loop reads the per-rhs stop state on the device, so the host only
+// polls the stop flags (double-buffered, one chunk behind) to stop launching.
+int e3x_dembx_advance(e3x_solver *h, int block, int *done)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    Solver::PcgRun &R = s.run;
+    if (!R.on) return fail(E3X_ESTATE, "e3x_dembx_advance without e3x_dembx_begin");
+    if (!done) return fail(E3X_EINVAL, "done is NULL");
+    *done = 0;
+    E3X_HIP(hipSetDevice(s.device));
     int *flags[2] = {s.flags_host, s.flags_host + kMaxRhs};
-    while (it <= ldemb) {
-        const int n = std::min(chunk, ldemb - it + 1);
-        for (int t = 0; t < n; t++) {
-            const int itt = it + t;
-            const bool replace = mixed && replace_every > 0 && itt % replace_every == 0;
-            const bool prev_replace = mixed && replace_every > 0 && itt > 1 && (itt - 1) % replace_every == 0;
-            E3X_HIP(launch_pcg_iter(s, mixed, itt, replace, prev_replace));
+    while (!R.finishing) {
+        // keep up to two chunks ahead of the last one examined
+        while (!R.stop_seen && R.it <= R.ldemb && R.k - R.checked < 2) {
+            const int n = std::min(R.chunk, R.ldemb - R.it + 1);
+            for (int t = 0; t < n; t++) {
+                const int itt = R.it + t;
+                const bool replace = R.replace_every > 0 && itt % R.replace_every == 0;
+                const bool prev_replace = R.replace_every > 0 && itt > 1 && (itt - 1) % R.replace_every == 0;
+                E3X_HIP(launch_pcg_iter(s, R.mixed, itt, replace, prev_replace));
+            }
+            R.it += n;
+            // the stop_iter of every rhs into the pinned slot of this chunk
+            for (int r = 0; r < s.nrhs; r++)
+                E3X_HIP(hipMemcpyAsync(flags[R.k & 1] + r, &s.st[r].stop_iter, sizeof(int), hipMemcpyDeviceToHost,
+                                       s.stream));
+            E3X_HIP(hipEventRecord(s.ev[2 + (R.k & 1)], s.stream));
+            R.k++;
         }
-        it += n;
-        // copy the stop_iter of every rhs into the pinned slot of this chunk
-        for (int r = 0; r < s.nrhs; r++)
-            E3X_HIP(hipMemcpyAsync(flags[k & 1] + r, &s.st[r].stop_iter, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-        E3X_HIP(hipEventRecord(s.ev[2 + (k & 1)], s.stream));
-        if (k > 0) {
-            E3X_HIP(hipEventSynchronize(s.ev[2 + ((k - 1) & 1)]));
+        if (R.checked < R.k) {  // examine the oldest chunk in flight
+            hipEvent_t ev = s.ev[2 + (R.checked & 1)];
+            if (block) {
+                E3X_HIP(hipEventSynchronize(ev));
+            } else {
+                const hipError_t q = hipEventQuery(ev);
+                if (q == hipErrorNotReady) return E3X_OK;
+                E3X_HIP(q);
+            }
             bool all = true;
-            for (int r = 0; r < s.nrhs; r++) all &= flags[(k - 1) & 1][r] != 0;
-            if (all) break;
+            for (int r = 0; r < s.nrhs; r++) all &= flags[R.checked & 1][r] != 0;
+            R.checked++;
+            if (all) R.stop_seen = true;
+            continue;
         }
-        k++;
+        // every right-hand side stopped, or the iteration limit is enqueued and examined:
+        // the last iteration's u step and direction update (deferred into the next apply)
+        E3X_HIP(launch_pcg_finalize(s, R.mixed, R.replace_every));
+        E3X_HIP(hipEventRecord(s.ev[2], s.stream));
+        if (R.mixed) {
+            const int64_t nv = (int64_t)s.nrhs * 3 * s.ns;
+            E3X_HIP(launch_to_f64(s.g32, s.g, nv, s.stream));
+            E3X_HIP(launch_to_f64(s.h32, s.h, nv, s.stream));
+        }
+        E3X_HIP(hipMemcpyAsync(s.st_host, s.st, sizeof(PcgState) * s.nrhs, hipMemcpyDeviceToHost, s.stream));
+        E3X_HIP(hipEventRecord(s.ev[3], s.stream));
+        R.finishing = true;
     }
-    // the last iteration's u step and direction update (deferred into the next apply)
-    E3X_HIP(launch_pcg_finalize(s, mixed, mixed ? replace_every : 0));
-    E3X_HIP(hipEventRecord(s.ev[2], s.stream));
-    if (mixed) {
-        E3X_HIP(launch_to_f64(s.g32, s.g, nv, s.stream));
-        E3X_HIP(launch_to_f64(s.h32, s.h, nv, s.stream));
+    if (block) {
+        E3X_HIP(hipEventSynchronize(s.ev[3]));
+    } else {
+        const hipError_t q = hipEventQuery(s.ev[3]);
+        if (q == hipErrorNotReady) return E3X_OK;
+        E3X_HIP(q);
     }
-    E3X_HIP(hipMemcpyAsync(s.st_host, s.st, sizeof(PcgState) * s.nrhs, hipMemcpyDeviceToHost, s.stream));
-    E3X_HIP(hipStreamSynchronize(s.stream));
+    *done = 1;
+    return E3X_OK;
+}
+
+int e3x_dembx_end(e3x_solver *h, int *lstep, double *gg, double *history, int *exit_flag, int *reset_count)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    Solver::PcgRun &R = s.run;
+    if (!R.on) return fail(E3X_ESTATE, "e3x_dembx_end without e3x_dembx_begin");
+    int done = 0;
+    if (int rc = e3x_dembx_advance(h, 1, &done)) return rc;
+    R.on = false;
+    const int ldemb = R.ldemb;
     float ms0 = 0, ms1 = 0;
     (void)hipEventElapsedTime(&ms0, s.ev[0], s.ev[1]);
     (void)hipEventElapsedTime(&ms1, s.ev[1], s.ev[2]);
@@ -376,15 +459,27 @@ int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *
         if (gg) gg[r] = st.gg;
         if (exit_flag) exit_flag[r] = st.exit_flag;
         if (reset_count) reset_count[r] = st.reset_count;
-        if (history && L > 0) {
-            E3X_HIP(hipMemcpy(history + (int64_t)r * ldemb, s.hist + (int64_t)r * ldemb, sizeof(double) * L,
-                              hipMemcpyDeviceToHost));
+        if (history) {
+            if (L > 0)
+                E3X_HIP(hipMemcpyAsync(history + (int64_t)r * ldemb, s.hist + (int64_t)r * ldemb,
+                                       sizeof(double) * L, hipMemcpyDeviceToHost, s.stream));
             for (int q = L; q < ldemb; q++) history[(int64_t)r * ldemb + q] = NAN;
-        } else if (history) {
-            for (int q = 0; q < ldemb; q++) history[(int64_t)r * ldemb + q] = NAN;
         }
     }
+    if (history) E3X_HIP(hipStreamSynchronize(s.stream));
     return E3X_OK;
+}
+
+int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *gg0, float tol, int ldemb,
+                 int kkk, int *lstep, double *gg, double *history, int *exit_flag, int *reset_count)
+{
+    if (int rc = e3x_dembx_begin(h, precision, replace_every, gg0, tol, ldemb, kkk)) return rc;
+    int done = 0;
+    if (int rc = e3x_dembx_advance(h, 1, &done)) {
+        S(h).run.on = false;
+        return rc;
+    }
+    return e3x_dembx_end(h, lstep, gg, history, exit_flag, reset_count);
 }
 
 int e3x_dembx(e3x_solver *h, const double *gg0, float tol, int ldemb, int kkk, int *lstep, double *gg,
@@ -395,7 +490,7 @@ int e3x_dembx(e3x_solver *h, const double *gg0, float tol, int ldemb, int kkk, i
 
 int e3x_stress(e3x_solver *h, double *stress6, double *strain6)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_femat) return fail(E3X_ESTATE, "e3x_stress before e3x_femat");
     E3X_HIP(hipSetDevice(s.device));
@@ -412,7 +507,7 @@ int e3x_stress(e3x_solver *h, double *stress6, double *strain6)
 
 int e3x_stress_fullfield(e3x_solver *h, int rhs, double *stress, double *vm)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_femat) return fail(E3X_ESTATE, "e3x_stress_fullfield before e3x_femat");
     if (rhs < 0 || rhs >= s.nrhs) return fail(E3X_EINVAL, "rhs out of range");
@@ -431,7 +526,7 @@ int e3x_effective_stiffness(e3x_solver *h, double c11, double c12, double c44, c
                             int flag_m, double magnitude, int precision, int replace_every, float tol,
                             int ldemb, double *C_eff, int *lstep, int *exit_flag)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (s.nrhs != 6) return fail(E3X_EINVAL, "e3x_effective_stiffness needs a handle with nrhs = 6");
     if (!C_eff) return fail(E3X_EINVAL, "C_eff is NULL");
@@ -481,7 +576,7 @@ int e3x_effective_stiffness(e3x_solver *h, double c11, double c12, double c44, c
 
 int e3x_get_field(e3x_solver *h, int which, int rhs, double *host)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!host) return fail(E3X_EINVAL, "host is NULL");
     if (rhs < 0 || rhs >= s.nrhs) return fail(E3X_EINVAL, "rhs out of range");
@@ -497,7 +592,7 @@ int e3x_get_field(e3x_solver *h, int which, int rhs, double *host)
 
 int e3x_set_field(e3x_solver *h, int which, int rhs, const double *host)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!host) return fail(E3X_EINVAL, "host is NULL");
     if (rhs < 0 || rhs >= s.nrhs) return fail(E3X_EINVAL, "rhs out of range");
@@ -513,7 +608,7 @@ int e3x_set_field(e3x_solver *h, int which, int rhs, const double *host)
 
 int e3x_get_tables(e3x_solver *h, double *cmod, double *dk)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_femat) return fail(E3X_ESTATE, "e3x_get_tables before e3x_femat");
     E3X_HIP(hipSetDevice(s.device));
@@ -534,7 +629,7 @@ int e3x_last_timing(e3x_solver *h, double *precond_ms, double *pcg_ms)
 
 int e3x_time_apply(e3x_solver *h, int precision, int reps, double *avg_ms)
 {
-    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!s.have_femat) return fail(E3X_ESTATE, "e3x_time_apply before e3x_femat");
     if (reps < 1 || !avg_ms) return fail(E3X_EINVAL, "reps must be >= 1 and avg_ms non-NULL");

@@ -41,6 +41,7 @@ enum StencilMode { kApply = 0, kEnergy = 1, kResid = 2, kPlain = 3, kApplyFused 
 
 struct Solver {
     int nx = 0, ny = 0, nz = 0, nphase = 0, nrhs = 1, device = 0;
+    int nphase_cap = 0;  // phases the tables are allocated for (nphase <= nphase_cap)
     int64_t ns = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
@@ -77,6 +78,14 @@ struct Solver {
     int *flags_host = nullptr;       // pinned stop flags
     std::vector<int64_t> counts;     // phase histogram of pix (assig)
 
+    // PCG in progress (e3x_dembx_begin / _advance / _end): chunks of iterations are
+    // enqueued ahead; the stop flags of chunk k land in flags_host[k & 1], marked by
+    // ev[2 + (k & 1)]; at most two chunks are unexamined at a time.
+    struct PcgRun {
+        bool on = false, mixed = false, stop_seen = false, finishing = false;
+        int replace_every = 0, ldemb = 0, chunk = 0, it = 1, k = 0, checked = 0;
+    } run;
+
     bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
     double last_precond_ms = 0, last_pcg_ms = 0;
     int64_t bytes = 0;
@@ -108,6 +117,13 @@ hipError_t launch_stress(Solver &s);                          // results[0..12*n
 hipError_t launch_fullfield(Solver &s, int rhs, double *stress_dev, double *vm_dev);
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
 hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st);
+
+// e3x_api.hip: reuse a handle for another microstructure with nphase <= nphase_cap
+// phases (the serving loop's handle pool); the microstructure and tables must be
+// loaded again.  Returns an e3x_status.
+int solver_set_nphase(Solver &s, int nphase);
+// e3x_api.hip: set the message e3x_last_error() returns on this thread; returns code
+int report_error(int code, const char *msg);
 
 int64_t stencil_blocks(const Solver &s);
 int64_t stencil_parts(const Solver &s);  // partials written per stencil launch (one per block)

@@ -1,0 +1,362 @@
+// Request loop over the gfx950 solver (include/e3x_serve.h): continuous batching of
+// solves of different grid sizes on one GPU.
+//
+// Each active request owns a solver handle, hence a HIP stream; its PCG advances in
+// chunks (e3x_dembx_advance without waiting), so the kernels of all active solves are
+// in flight together and small grids fill the GPU side by side.  Admission is by the
+// device bytes a handle needs (first fit in arrival order under the budget and the slot
+// limit); a finished handle stays in a pool for a later request of the same grid size
+// (freeing device memory synchronizes the device, so it is done only to make room).
+#include "e3x.h"
+#include "e3x_serve.h"
+#include "e3x_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace e3x;
+
+namespace {
+
+// Phases a serving handle is created for: any request with at most this many (grains
+// plus the pore phase) can reuse it.
+constexpr int kPoolPhases = 1024;
+
+struct Request {
+    int64_t id = 0;
+    e3x_request rq{};
+    std::vector<int32_t> pix;
+    std::vector<double> ori;
+    double t_submit = 0;
+};
+
+struct Handle {
+    e3x_solver *h = nullptr;
+    int nx = 0, ny = 0, nz = 0, nphase_cap = 0;
+    int64_t bytes = 0;
+};
+
+struct Active {
+    Request req;
+    Handle hd;
+    double t_admit = 0, utot0 = 0;
+};
+
+struct Server {
+    int device = 0, max_active = 1;
+    int64_t budget = 0, used = 0;
+    std::chrono::steady_clock::time_point t0;
+    int64_t next_id = 0;
+    std::deque<Request> queue;
+    std::vector<Active> active;
+    std::list<Handle> pool;  // idle handles, least recently used first
+    std::deque<e3x_response> done;
+
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+Server &SV(e3x_server *p) { return *reinterpret_cast<Server *>(p); }
+const Server &SV(const e3x_server *p) { return *reinterpret_cast<const Server *>(p); }
+
+// Device bytes of a one-rhs handle created for nphase_cap phases, with the mixed form's
+// buffers and an iteration history of ldemb (the allocations of e3x_create,
+// ensure_mixed_buffers and ensure_hist).
+int64_t footprint(int nx, int ny, int nz, int nphase_cap, bool mixed, int ldemb)
+{
+    const int64_t ns = (int64_t)nx * ny * nz;
+    int64_t b = ns * 2;                                   // pix
+    b += 7 * 3 * ns * 8 + 6 * ns * 8;                     // u g b h h_alt z Ah, M^-1
+    b += (int64_t)nphase_cap * ((36 + 576 + kStenHalf + 3) * 8 + (576 + kStenHalf) * 4);
+    b += std::max<int64_t>(12 * 2048, 2 * ((ns + 255) / 256)) * 8 + (1 << 16);  // partials, state
+    b += (int64_t)std::max(ldemb, 1) * 8;
+    if (mixed) b += 5 * 3 * ns * 4 + 6 * ns * 4;
+    return b;
+}
+
+int64_t handle_bytes(e3x_solver *h)
+{
+    int64_t b = 0;
+    (void)e3x_memory_bytes(h, &b);
+    return b;
+}
+
+void retire_handle(Server &S, Handle hd)
+{
+    const int64_t now_bytes = handle_bytes(hd.h);  // grows by the mixed buffers on first use
+    S.used += now_bytes - hd.bytes;
+    hd.bytes = now_bytes;
+    S.pool.push_back(hd);
+}
+
+// An idle handle of this grid size and enough phases, or one made room for and created.
+int acquire_handle(Server &S, const e3x_request &rq, int64_t need, Handle *out, bool *fits)
+{
+    *fits = true;
+    for (auto it = S.pool.begin(); it != S.pool.end(); ++it)
+        if (it->nx == rq.nx && it->ny == rq.ny && it->nz == rq.nz && it->nphase_cap >= rq.nphase) {
+            *out = *it;
+            S.pool.erase(it);
+            return E3X_OK;
+        }
+    // free idle handles of other sizes, least recently used first, until it fits
+    while (S.used + need > S.budget && !S.pool.empty()) {
+        S.used -= S.pool.front().bytes;
+        (void)e3x_destroy(S.pool.front().h);
+        S.pool.pop_front();
+    }
+    if (S.used + need > S.budget) {
+        *fits = false;
+        return E3X_OK;
+    }
+    Handle hd;
+    hd.nx = rq.nx; hd.ny = rq.ny; hd.nz = rq.nz;
+    hd.nphase_cap = std::max(kPoolPhases, rq.nphase);
+    if (int rc = e3x_create(rq.nx, rq.ny, rq.nz, hd.nphase_cap, 1, S.device, &hd.h)) return rc;
+    hd.bytes = handle_bytes(hd.h);
+    S.used += hd.bytes;
+    *out = hd;
+    return E3X_OK;
+}
+
+e3x_response make_response(const Request &r, int status)
+{
+    e3x_response o;
+    std::memset(&o, 0, sizeof(o));
+    o.id = r.id;
+    o.status = status;
+    o.nx = r.rq.nx; o.ny = r.rq.ny; o.nz = r.rq.nz;
+    o.t_submit = r.t_submit;
+    return o;
+}
+
+// The main program's sequence up to the PCG entry (f90:380-473) on the handle.
+int start_solve(Active &a)
+{
+    const e3x_request &q = a.req.rq;
+    e3x_solver *h = a.hd.h;
+    if (int rc = solver_set_nphase(*reinterpret_cast<Solver *>(h), q.nphase)) return rc;
+    if (int rc = e3x_set_microstructure(h, a.req.pix.data(), a.req.ori.empty() ? nullptr : a.req.ori.data()))
+        return rc;
+    double cval = 0;
+    if (int rc = e3x_femat(h, q.c11, q.c12, q.c44, q.phasemod, q.flag_m, q.strain6, &cval)) return rc;
+    if (int rc = e3x_affine_guess(h)) return rc;
+    double gg0 = 0;
+    if (int rc = e3x_energy(h, &a.utot0, &gg0)) return rc;
+    return e3x_dembx_begin(h, q.precision, q.replace_every, &gg0, q.tol, q.ldemb, 1);
+}
+
+// After the PCG (f90:479-519): energy and averages.
+int finish_solve(Active &a, e3x_response &o)
+{
+    e3x_solver *h = a.hd.h;
+    if (int rc = e3x_dembx_end(h, &o.lstep, &o.gg, nullptr, &o.exit_flag, &o.reset_count)) return rc;
+    double gg = 0;
+    if (int rc = e3x_energy(h, &o.utot, &gg)) return rc;
+    if (int rc = e3x_stress(h, o.stress6, o.strain6)) return rc;
+    o.utot0 = a.utot0;
+    return E3X_OK;
+}
+
+int progress(Server &S, bool *changed)
+{
+    *changed = false;
+    // retire the solves whose PCG has finished on the device
+    for (size_t i = 0; i < S.active.size();) {
+        Active &a = S.active[i];
+        int fin = 0;
+        int rc = e3x_dembx_advance(a.hd.h, 0, &fin);
+        if (rc == E3X_OK && !fin) {
+            ++i;
+            continue;
+        }
+        e3x_response o = make_response(a.req, rc);
+        if (rc == E3X_OK) o.status = finish_solve(a, o);
+        else (void)e3x_dembx_end(a.hd.h, nullptr, nullptr, nullptr, nullptr, nullptr);
+        o.t_admit = a.t_admit;
+        o.t_done = S.now();
+        S.done.push_back(o);
+        retire_handle(S, a.hd);
+        S.active.erase(S.active.begin() + (std::ptrdiff_t)i);
+        *changed = true;
+    }
+    // admit queued requests: first fit in arrival order
+    for (auto it = S.queue.begin(); it != S.queue.end() && (int)S.active.size() < S.max_active;) {
+        const e3x_request &q = it->rq;
+        const int64_t need = footprint(q.nx, q.ny, q.nz, std::max(kPoolPhases, q.nphase),
+                                       q.precision == E3X_PREC_MIXED, q.ldemb);
+        Handle hd;
+        bool fits = true;
+        if (int rc = acquire_handle(S, q, need, &hd, &fits)) {
+            e3x_response o = make_response(*it, rc);
+            o.t_admit = o.t_done = S.now();
+            S.done.push_back(o);
+            it = S.queue.erase(it);
+            *changed = true;
+            continue;
+        }
+        if (!fits) {
+            ++it;
+            continue;
+        }
+        Active a;
+        a.req = std::move(*it);
+        a.hd = hd;
+        a.t_admit = S.now();
+        it = S.queue.erase(it);
+        *changed = true;
+        if (int rc = start_solve(a)) {
+            e3x_response o = make_response(a.req, rc);
+            o.t_admit = a.t_admit;
+            o.t_done = S.now();
+            S.done.push_back(o);
+            if (reinterpret_cast<Solver *>(a.hd.h)->run.on)
+                (void)e3x_dembx_end(a.hd.h, nullptr, nullptr, nullptr, nullptr, nullptr);
+            retire_handle(S, a.hd);
+            continue;
+        }
+        S.active.push_back(std::move(a));
+    }
+    // keep every active PCG supplied with iteration chunks
+    for (Active &a : S.active) {
+        int fin = 0;
+        (void)e3x_dembx_advance(a.hd.h, 0, &fin);
+    }
+    return E3X_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int e3x_server_create(int device, int64_t hbm_budget_bytes, int max_active, e3x_server **out)
+{
+    if (!out) return report_error(E3X_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (max_active < 1 || max_active > 64) return report_error(E3X_EINVAL, "max_active must be in [1, 64]");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+        return report_error(E3X_EINVAL, "no such HIP device");
+    if (hipSetDevice(device) != hipSuccess) return report_error(E3X_EDEVICE, "hipSetDevice failed");
+    int64_t budget = hbm_budget_bytes;
+    if (budget <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return report_error(E3X_EDEVICE, "hipMemGetInfo failed");
+        budget = (int64_t)(0.8 * (double)fr);
+    }
+    Server *s = new (std::nothrow) Server;
+    if (!s) return report_error(E3X_ENOMEM, "host allocation failed");
+    s->device = device;
+    s->max_active = max_active;
+    s->budget = budget;
+    s->t0 = std::chrono::steady_clock::now();
+    *out = reinterpret_cast<e3x_server *>(s);
+    return E3X_OK;
+}
+
+int e3x_server_destroy(e3x_server *sv)
+{
+    if (!sv) return E3X_OK;
+    Server &S = SV(sv);
+    for (Active &a : S.active) {
+        (void)e3x_dembx_end(a.hd.h, nullptr, nullptr, nullptr, nullptr, nullptr);
+        (void)e3x_destroy(a.hd.h);
+    }
+    for (Handle &hd : S.pool) (void)e3x_destroy(hd.h);
+    delete &S;
+    return E3X_OK;
+}
+
+int e3x_server_submit(e3x_server *sv, const e3x_request *rq, int64_t *id)
+{
+    if (!sv || !rq) return report_error(E3X_EINVAL, "NULL server or request");
+    Server &S = SV(sv);
+    if (rq->nx < 2 || rq->ny < 2 || rq->nz < 2) return report_error(E3X_EINVAL, "grid extents must be >= 2");
+    if ((int64_t)rq->nx * rq->ny * rq->nz > (int64_t)INT32_MAX) return report_error(E3X_EINVAL, "grid too large");
+    if (rq->nphase < 1 || rq->nphase > 65535) return report_error(E3X_EINVAL, "nphase must be in [1, 65535]");
+    if (!rq->pix) return report_error(E3X_EINVAL, "pix is NULL");
+    if (rq->flag_m != 0 && rq->nphase > 1 && !rq->orientation)
+        return report_error(E3X_EINVAL, "orientation is NULL");
+    const int64_t need = footprint(rq->nx, rq->ny, rq->nz, std::max(kPoolPhases, rq->nphase),
+                                   rq->precision == E3X_PREC_MIXED, rq->ldemb);
+    if (need > S.budget) return report_error(E3X_ENOMEM, "request exceeds the server's memory budget");
+    Request r;
+    r.id = S.next_id++;
+    r.rq = *rq;
+    const int64_t ns = (int64_t)rq->nx * rq->ny * rq->nz;
+    r.pix.assign(rq->pix, rq->pix + ns);
+    if (rq->orientation && rq->nphase > 1) r.ori.assign(rq->orientation, rq->orientation + 3 * (rq->nphase - 1));
+    r.rq.pix = nullptr;
+    r.rq.orientation = nullptr;
+    r.t_submit = S.now();
+    S.queue.push_back(std::move(r));
+    if (id) *id = S.next_id - 1;
+    return E3X_OK;
+}
+
+int e3x_server_progress(e3x_server *sv, int *n_active, int *n_queued, int *n_done)
+{
+    if (!sv) return report_error(E3X_EINVAL, "NULL server");
+    Server &S = SV(sv);
+    if (hipSetDevice(S.device) != hipSuccess) return report_error(E3X_EDEVICE, "hipSetDevice failed");
+    bool changed = false;
+    if (int rc = progress(S, &changed)) return rc;
+    if (n_active) *n_active = (int)S.active.size();
+    if (n_queued) *n_queued = (int)S.queue.size();
+    if (n_done) *n_done = (int)S.done.size();
+    return E3X_OK;
+}
+
+int e3x_server_take(e3x_server *sv, e3x_response *out, int *got)
+{
+    if (!sv || !out || !got) return report_error(E3X_EINVAL, "NULL argument");
+    Server &S = SV(sv);
+    *got = 0;
+    if (S.done.empty()) return E3X_OK;
+    *out = S.done.front();
+    S.done.pop_front();
+    *got = 1;
+    return E3X_OK;
+}
+
+int e3x_server_drain(e3x_server *sv)
+{
+    if (!sv) return report_error(E3X_EINVAL, "NULL server");
+    Server &S = SV(sv);
+    if (hipSetDevice(S.device) != hipSuccess) return report_error(E3X_EDEVICE, "hipSetDevice failed");
+    while (!S.queue.empty() || !S.active.empty()) {
+        bool changed = false;
+        if (int rc = progress(S, &changed)) return rc;
+        if (!changed) std::this_thread::sleep_for(std::chrono::microseconds(100));
+        if (!changed && S.active.empty() && !S.queue.empty())
+            return report_error(E3X_ENOMEM, "queued requests cannot be admitted under the memory budget");
+    }
+    return E3X_OK;
+}
+
+int e3x_server_clock(const e3x_server *sv, double *seconds)
+{
+    if (!sv || !seconds) return report_error(E3X_EINVAL, "NULL argument");
+    *seconds = SV(sv).now();
+    return E3X_OK;
+}
+
+int e3x_server_memory(const e3x_server *sv, int64_t *bytes, int *handles)
+{
+    if (!sv) return report_error(E3X_EINVAL, "NULL server");
+    const Server &S = SV(sv);
+    if (bytes) *bytes = S.used;
+    if (handles) *handles = (int)(S.active.size() + S.pool.size());
+    return E3X_OK;
+}
+
+}  // extern "C"

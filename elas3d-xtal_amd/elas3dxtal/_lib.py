@@ -46,6 +46,9 @@ SIGNATURES = {
     "e3x_dembx": (C.c_int, [_vp, _dp, C.c_float, C.c_int, C.c_int, _ip, _dp, _dp, _ip, _ip]),
     "e3x_dembx_ex": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_float, C.c_int, C.c_int, _ip, _dp,
                                _dp, _ip, _ip]),
+    "e3x_dembx_begin": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_float, C.c_int, C.c_int]),
+    "e3x_dembx_advance": (C.c_int, [_vp, C.c_int, _ip]),
+    "e3x_dembx_end": (C.c_int, [_vp, _ip, _dp, _dp, _ip, _ip]),
     "e3x_stress": (C.c_int, [_vp, _dp, _dp]),
     "e3x_effective_stiffness": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_double,
                                           C.c_int, C.c_int, C.c_float, C.c_int, _dp, _ip, _ip]),
@@ -55,6 +58,38 @@ SIGNATURES = {
     "e3x_get_tables": (C.c_int, [_vp, _dp, _dp]),
     "e3x_last_timing": (C.c_int, [_vp, _dp, _dp]),
     "e3x_time_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+}
+
+
+
+class Request(C.Structure):
+    """struct e3x_request (include/e3x_serve.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nphase", C.c_int),
+                ("pix", _i32p), ("orientation", _dp),
+                ("c11", C.c_double), ("c12", C.c_double), ("c44", C.c_double),
+                ("phasemod", C.c_double * 4), ("flag_m", C.c_int), ("strain6", C.c_double * 6),
+                ("precision", C.c_int), ("replace_every", C.c_int), ("tol", C.c_float), ("ldemb", C.c_int)]
+
+
+class Response(C.Structure):
+    """struct e3x_response (include/e3x_serve.h)."""
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("lstep", C.c_int), ("exit_flag", C.c_int),
+                ("reset_count", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("utot0", C.c_double), ("utot", C.c_double), ("gg", C.c_double),
+                ("stress6", C.c_double * 6), ("strain6", C.c_double * 6),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+# mirrors include/e3x_serve.h
+SERVE_SIGNATURES = {
+    "e3x_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(_vp)]),
+    "e3x_server_destroy": (C.c_int, [_vp]),
+    "e3x_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "e3x_server_progress": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "e3x_server_take": (C.c_int, [_vp, C.POINTER(Response), _ip]),
+    "e3x_server_drain": (C.c_int, [_vp]),
+    "e3x_server_clock": (C.c_int, [_vp, _dp]),
+    "e3x_server_memory": (C.c_int, [_vp, C.POINTER(C.c_int64), _ip]),
 }
 
 _lib = None
@@ -69,7 +104,7 @@ def load():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                            "(the HIP library is required; there is no CPU fallback)")
     lib = C.CDLL(LIB_PATH)
-    for name, (res, args) in SIGNATURES.items():
+    for name, (res, args) in list(SIGNATURES.items()) + list(SERVE_SIGNATURES.items()):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args

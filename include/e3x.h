@@ -148,6 +148,21 @@ int e3x_dembx_ex(e3x_solver *s, int precision, int replace_every, const double *
                  int ldemb, int kkk, int *lstep, double *gg, double *history, int *exit_flag,
                  int *reset_count);
 
+/*
+ * e3x_dembx_ex in three steps, so that one host thread can keep several handles'
+ * solves in flight on the GPU at once (each handle has its own stream); no counterpart
+ * in the reference.  e3x_dembx_begin validates, builds the preconditioner and enqueues
+ * the PCG entry; e3x_dembx_advance enqueues further iteration chunks and sets *done = 1
+ * once the loop and its final updates have completed on the device (block = 0: returns
+ * at once if the GPU is still working; block = 1: waits); e3x_dembx_end returns what
+ * e3x_dembx_ex returns (it waits for completion if needed).  No other call on the handle
+ * between begin and end.
+ */
+int e3x_dembx_begin(e3x_solver *s, int precision, int replace_every, const double *gg0, float tol, int ldemb,
+                    int kkk);
+int e3x_dembx_advance(e3x_solver *s, int block, int *done);
+int e3x_dembx_end(e3x_solver *s, int *lstep, double *gg, double *history, int *exit_flag, int *reset_count);
+
 /* stress_OpenMP (f90:2149-2408): stress6[6*nrhs], strain6[6*nrhs] volume averages. */
 int e3x_stress(e3x_solver *s, double *stress6, double *strain6);
 

@@ -25,6 +25,21 @@ def test_core_library_exports_header_symbols():
     assert set(names) == set(L.SIGNATURES), "binding and header disagree"
 
 
+def test_serving_library_exports_header_symbols():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    names = declared("e3x_serve.h")
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(L.SERVE_SIGNATURES), "binding and header disagree"
+
+
+def test_request_response_layout_matches_header():
+    """The ctypes structs have the C layout (offsets of the last members)."""
+    assert ctypes.sizeof(L.Request) == 4 * 4 + 2 * 8 + 3 * 8 + 4 * 8 + 4 + 4 + 6 * 8 + 4 + 4 + 4 + 4
+    assert L.Response.t_done.offset == ctypes.sizeof(L.Response) - 8
+
+
 def test_version_and_error_string_without_gpu():
     lib = L.load()
     assert lib.e3x_version() >= 10000

@@ -1,0 +1,111 @@
+"""The request loop (include/e3x_serve.h) on the GPU: every result is the same served
+in a mix of concurrent solves as alone, under memory-based admission, with requests
+arriving while others run."""
+import numpy as np
+import pytest
+
+import _oracle as oc
+from _cases import voronoi_case
+from elas3dxtal import _lib as L
+from elas3dxtal.serve import Server
+from elas3dxtal.solver import solve
+
+pytestmark = pytest.mark.gpu
+
+C12, C44 = 147.3e9, 124.7e9
+
+
+def _requests(n, seed):
+    """Seeded requests of different grid sizes, grains, Zener ratios, precisions and load
+    cases (one unit macro strain each)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        shape = tuple(int(v) for v in rng.choice([6, 8, 10, 12, 16], 3))
+        ng = int(rng.integers(2, 9))
+        pix, rod = voronoi_case(shape, ng, seed=seed * 100 + k, pore_frac=float(rng.uniform(0, 0.05)))
+        a = float(rng.uniform(0.5, 5.0))
+        e6 = np.zeros(6)
+        j = int(rng.integers(0, 6))
+        e6[j] = oc.AML_F32 if j < 3 else 0.5 * oc.AML_F32
+        prec = L.PREC_MIXED if k % 2 else L.PREC_FP64
+        out.append(dict(pix=pix, rod=rod, shape=shape, nphase=ng + 1, c11=C12 + 2 * C44 / a, e6=e6, prec=prec))
+    return out
+
+
+def _alone(q):
+    r = solve(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], C12, C44, q["e6"], precision=q["prec"])
+    return dict(lstep=int(r.lstep[0]), utot=float(r.utot[0]), stress6=r.stress6[0], exit=int(r.exit[0]))
+
+
+def _submit(sv, q):
+    return sv.submit(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], C12, C44, q["e6"], precision=q["prec"])
+
+
+def _check_same(served, alone):
+    assert served["status"] == L.E3X_OK
+    assert served["exit"] == alone["exit"] == L.PCG_CONVERGED
+    assert served["lstep"] == alone["lstep"]
+    # fixed-order reductions: the same bits whatever runs beside it
+    assert served["utot"] == alone["utot"]
+    assert np.array_equal(served["stress6"], alone["stress6"])
+
+
+def test_served_results_equal_solves_alone():
+    qs = _requests(8, seed=11)
+    with Server(max_active=4) as sv:
+        ids = [_submit(sv, q) for q in qs]
+        sv.drain()
+        got = {r["id"]: r for r in sv.take()}
+    assert sorted(got) == ids
+    for i, q in zip(ids, qs):
+        _check_same(got[i], _alone(q))
+        assert got[i]["t_submit"] <= got[i]["t_admit"] <= got[i]["t_done"]
+
+
+def _handle_bytes(shape, mixed, ldemb=10000, phases=1024):
+    """Device bytes the server books for a one-rhs handle (its admission estimate)."""
+    ns = int(np.prod(shape))
+    b = ns * 2 + 7 * 3 * ns * 8 + 6 * ns * 8
+    b += phases * ((36 + 576 + 126 + 3) * 8 + (576 + 126) * 4)
+    b += max(12 * 2048, 2 * ((ns + 255) // 256)) * 8 + (1 << 16) + ldemb * 8
+    if mixed:
+        b += 5 * 3 * ns * 4 + 6 * ns * 4
+    return b
+
+
+def test_admission_by_memory_and_arrivals_during_work():
+    """A budget that holds two solver handles: two of three requests start, the third
+    waits; three more arrive while the first ones run; handles are reused."""
+    qs = [dict(q, shape=(12, 12, 12)) for q in _requests(6, seed=12)]
+    for k, q in enumerate(qs):
+        q["pix"], q["rod"] = voronoi_case(q["shape"], q["nphase"] - 1, seed=1200 + k, pore_frac=0.02)
+    budget = int(2.5 * _handle_bytes((12, 12, 12), mixed=True))
+    with Server(hbm_budget=budget, max_active=6) as sv:
+        ids = [_submit(sv, q) for q in qs[:3]]
+        active, queued, _ = sv.progress()
+        assert (active, queued) == (2, 1)
+        while sv.progress()[2] == 0:          # until one has finished
+            pass
+        ids += [_submit(sv, q) for q in qs[3:]]
+        sv.drain()
+        got = {r["id"]: r for r in sv.take()}
+        used, handles = sv.memory()
+    assert handles == 2 and used <= budget
+    assert sorted(got) == ids
+    for i, q in zip(ids, qs):
+        _check_same(got[i], _alone(q))
+
+
+def test_bad_request_is_reported_not_fatal():
+    qs = _requests(2, seed=13)
+    bad = dict(qs[0])
+    bad["pix"] = bad["pix"].copy()
+    bad["pix"][0] = bad["nphase"] + 1            # label out of range (f90:2986-2989)
+    with Server(max_active=2) as sv:
+        i_bad = _submit(sv, bad)
+        i_ok = _submit(sv, qs[1])
+        sv.drain()
+        got = {r["id"]: r for r in sv.take()}
+    assert got[i_bad]["status"] == L.E3X_EPHASE
+    _check_same(got[i_ok], _alone(qs[1]))
