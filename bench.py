@@ -14,8 +14,9 @@ PCG loop, HIP events on the solver's stream), summed over the K timed steps.
 Also reported: solves/s and per-solve latency percentiles (host wall clock), the
 same for the mixed-precision form, the six unit load cases of the effective stiffness
 as six right-hand sides of one PCG (FP64 and mixed; each right-hand side counts as a
-solve), the roofline of the stencil-apply kernel, and the CPU baseline (the oracle's
-C/OpenMP restatement of the reference on this host).
+solve), a serving stream of mixed-size requests with Poisson arrivals (continuous
+batching: solves/s and p50/p99 latency), the roofline of the stencil-apply kernel, and
+the CPU baseline (the oracle's C/OpenMP restatement of the reference on this host).
 """
 import argparse
 import ctypes as C
@@ -111,6 +112,69 @@ def run_six(w, c11, c12, c44, steps, warmup, precision):
         s.close()
 
 
+def run_serving(n_requests, interarrival_ms, max_active, seed=5000):
+    """Serving stream: requests with Poisson arrivals, grid edge uniform over
+    {64, 96, 128, 160, 200}, 10-800 grains, Zener ratio A uniform on [0.5, 5], porosity
+    uniform on [0, 5] % as lognormal spheres, one random unit macro strain (1e-3) each,
+    mixed precision with replacement every 50 iterations.  The microstructures come
+    from a pool of two per grid edge, generated before the timed stream (host
+    generation of one per request would dominate it); every request draws its own pool
+    entry, A and load case.  One host thread submits at the arrival times and drives the
+    server between them."""
+    from elas3dxtal import io
+    from elas3dxtal.serve import Server
+
+    rng = np.random.default_rng(seed)
+    edges = [64, 96, 128, 160, 200]
+    pool = []
+    for e in edges:
+        for rep in range(2):
+            ng = int(rng.integers(10, 801))
+            frac = float(rng.uniform(0.0, 0.05))
+            pix, rod, _, _ = io.generate((e, e, e), ng, 1000 * 5 + len(pool), pore_mode=io.PORES_SPHERES,
+                                         pore_fraction=frac, pore_median=3.0, pore_sigma=0.4)
+            pool.append(dict(shape=(e, e, e), nphase=ng + 1, pix=pix, rod=rod))
+    reqs = []
+    for _ in range(n_requests):
+        m = pool[int(rng.integers(0, len(pool)))]
+        a = float(rng.uniform(0.5, 5.0))
+        e6 = np.zeros(6)
+        j = int(rng.integers(0, 6))
+        e6[j] = 1e-3 if j < 3 else 0.5e-3
+        reqs.append((m, (configs.C12_GPA + 2.0 * configs.C44_GPA / a) * 1e9, e6))
+    gaps = rng.exponential(interarrival_ms * 1e-3, n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    with Server(max_active=max_active) as sv:
+        t0 = time.perf_counter()
+        done = []
+        k = 0
+        while len(done) < n_requests:
+            now = time.perf_counter() - t0
+            while k < n_requests and arrivals[k] <= now:
+                m, c11, e6 = reqs[k]
+                sv.submit(m["pix"], m["rod"], m["shape"], m["nphase"], c11, configs.C12_GPA * 1e9, configs.C44_GPA * 1e9, e6,
+                          precision=L.PREC_MIXED, replace_every=50)
+                k += 1
+            sv.progress()
+            done += sv.take()
+            if k < n_requests and not done:
+                time.sleep(1e-4)
+        wall = time.perf_counter() - t0
+    lat = np.array([r["t_done"] - r["t_submit"] for r in done])
+    ok = sum(r["status"] == 0 and r["exit"] == L.PCG_CONVERGED for r in done)
+    dofups = sum(3.0 * np.prod(r["shape"]) * r["lstep"] for r in done)
+    return {
+        "requests": n_requests, "converged": int(ok), "max_active": max_active,
+        "mean_interarrival_ms": interarrival_ms,
+        "solves_per_s": n_requests / wall,
+        "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
+        "latency_ms_p99": float(np.percentile(lat, 99) * 1e3),
+        "dof_updates_per_s": float(dofups / wall),
+        "wall_s": wall,
+        "pool": "2 microstructures per grid edge {64,96,128,160,200}, 10-800 grains, 0-5 % pores",
+    }
+
+
 def cpu_baseline(w, c11, c12, c44, e6, iters):
     """The oracle (C/OpenMP restatement of the reference) on this host: `iters` PCG
     iterations on the same 200^3 workload, timed around oc_dembx only."""
@@ -176,6 +240,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-six", action="store_true", help="skip the six-load-case (effective stiffness) runs")
     ap.add_argument("--six-steps", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0: skip)")
+    ap.add_argument("--serve-interarrival-ms", type=float, default=70.0)
+    ap.add_argument("--serve-max-active", type=int, default=8)
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run of the apply kernel and a calibration copy, for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -266,6 +333,8 @@ def main():
         six["ceff_mixed_rel_err_vs_fp64"] = float(np.max(np.abs(ce32 - ce64)) / np.max(np.abs(ce64)))
         six["mixed_six_vs_fp64_single_solves_ratio"] = six["mixed"]["solves_per_s"] / out["solves_per_s"]
         out["six_rhs"] = six
+    if args.serve_requests > 0:
+        out["serving"] = run_serving(args.serve_requests, args.serve_interarrival_ms, args.serve_max_active)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, c11, c12, c44, e6, args.cpu_iters)
     else:
