@@ -43,7 +43,7 @@ hipError_t dalloc(Solver &s, T **p, int64_t n)
 void release(Solver &s)
 {
     (void)hipSetDevice(s.device);
-    void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.u, s.g, s.b, s.h, s.h_alt, s.z,
+    void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.perm, s.u, s.g, s.b, s.h, s.h_alt, s.z,
                    s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
                    s.st, s.hist, s.results, s.post};
     for (void *p : dev)
@@ -176,6 +176,7 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->sten32, (int64_t)nphase * kStenHalf));
     CK(dalloc(*s, &s->orient, (int64_t)3 * std::max(nphase - 1, 1)));
     CK(dalloc(*s, &s->pix, ns));
+    CK(dalloc(*s, &s->perm, stencil_blocks(*s) * kSB));
     CK(dalloc(*s, &s->u, nv));
     CK(dalloc(*s, &s->g, nv));
     CK(dalloc(*s, &s->b, nv));
@@ -246,6 +247,7 @@ int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orie
     if (orientation && s.nphase > 1)
         E3X_HIP(hipMemcpyAsync(s.orient, orientation, sizeof(double) * 3 * (size_t)(s.nphase - 1),
                                hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(launch_tile_order(s));
     E3X_HIP(hipStreamSynchronize(s.stream));
     s.have_pix = true;
     s.have_femat = s.have_precond = s.have_precond32 = false;

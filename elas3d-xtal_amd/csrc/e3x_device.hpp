@@ -88,11 +88,11 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *partials
 // leaves at once instead of waiting for the slowest wave of its block.  `arrived` is a
 // block-shared counter set to 0 before a __syncthreads() that precedes this call; every
 // lane of every wave calls it (all lanes active).
-template <int NV>
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void write_partials_nowait(double (&v)[NV], double *partials, const int64_t (&slot)[NV],
                                                       unsigned *arrived)
 {
-    constexpr int NW = kBlock / 64;
+    constexpr int NW = NT / 64;
     __shared__ double sh[NV][NW];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     double t[NV];

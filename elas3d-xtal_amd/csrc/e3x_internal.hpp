@@ -13,8 +13,10 @@ constexpr int kMaxRhs = 6;
 constexpr int kBlock = 256;        // threads per block of every kernel
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
 
-// Stencil tile: 8 x 8 x 4 nodes per block; each 64-lane wave owns a 4x4x4 sub-cube.
+// Stencil tile: 8 x 8 x 4 nodes per block of kSB threads; each 64-lane wave owns a
+// 4x4x4 sub-cube.  (8 x 8 x 8 tiles, 512 threads, ran 8 % slower at 200^3.)
 constexpr int TX = 8, TY = 8, TZ = 4;
+constexpr int kSB = TX * TY * TZ;  // threads of a stencil block
 constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HN = HX * HY * HZ;  // node halo
 constexpr int VX = TX + 1, VY = TY + 1, VZ = TZ + 1, VN = VX * VY * VZ;  // voxel halo
 // Homogeneous 27-point stencil per phase, symmetric half: blocks of neighbours 14..26
@@ -53,6 +55,10 @@ struct Solver {
     float *dk32 = nullptr, *sten32 = nullptr;
     double *orient = nullptr;                                  // [3*(nphase-1)]
     uint16_t *pix = nullptr;                                   // 0-based phase per voxel
+    // per stencil tile, the order in which the block's lanes take its nodes (node t of
+    // the tile's wave-cube numbering): homogeneous nodes by phase, then mixed nodes by
+    // their phases, so that waves hold few distinct phases; built on upload
+    uint8_t *perm = nullptr;                                   // [tiles][kSB]
 
     // fields, [nrhs][3][ns]
     double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;
@@ -105,6 +111,7 @@ hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K 
 hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u);
 hipError_t launch_apply_only(Solver &s, bool mixed);          // fused apply with the current state
 hipError_t launch_resid(Solver &s);                           // g32 = K u + b in FP64
+hipError_t launch_tile_order(Solver &s);                      // perm from pix
 // e3x_pcg.hip
 hipError_t launch_hah_finish(Solver &s, int nparts, int it);
 hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,

@@ -3,9 +3,9 @@
 //
 // A block owns an 8x8x4 tile of nodes and stages the 10x10x6 node halo of x and the
 // 9x9x5 voxel phases in LDS.  A node is homogeneous when its 8 voxels share one phase,
-// mixed otherwise.  After staging, the block packs its homogeneous nodes into the
-// leading lanes and its mixed nodes into the trailing ones (a stable ballot
-// compaction), so that most waves hold one kind only.  Coefficients are wave-uniform
+// mixed otherwise.  The lanes take the tile's nodes in an order fixed when the
+// microstructure is loaded (k_tile_order): homogeneous nodes first, by phase, then the
+// mixed ones by their phases, so that most waves hold one kind and few phases.  Coefficients are wave-uniform
 // and read through the scalar cache (SGPR operands, no per-lane coefficient loads):
 //   - homogeneous lanes: the wave loops over the phases its lanes hold; for phase P the
 //     27-point stencil of P (243 FMA; the block of -delta is the transpose of the block
@@ -30,6 +30,7 @@ struct StencilArgs {
     int nx, ny, nz, ntx, nty, nrhs;
     int64_t ns;
     const uint16_t *pix;
+    const uint8_t *perm;   // lane order per tile (Solver::perm)
     const T *dk, *sten;
     const T *x;            // [nrhs][3][ns]
     T *y;                  // [nrhs][3][ns] (kApply, kEnergy, kPlain)
@@ -46,6 +47,11 @@ struct StencilArgs {
 };
 
 __device__ __forceinline__ int pmod(int v, int n) { v %= n; return v < 0 ? v + n : v; }
+
+// Wave w of a block owns the 4x4x4 cube at (wave_x, wave_y, wave_z) of the tile.
+__device__ __forceinline__ int wave_x(int w) { return (w % (TX / 4)) * 4; }
+__device__ __forceinline__ int wave_y(int w) { return ((w / (TX / 4)) % (TY / 4)) * 4; }
+__device__ __forceinline__ int wave_z(int w) { return (w / ((TX / 4) * (TY / 4))) * 4; }
 
 // The phase tables are read-only while a stencil kernel runs.  Reading them through the
 // constant address space makes every wave-uniform coefficient load a scalar load
@@ -79,8 +85,6 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     constexpr int NP = MODE == kEnergy ? 2 : 1;  // partial sums per right-hand side
     __shared__ T hs[RB * 3 * HN];
     __shared__ int ps[VN];
-    __shared__ int cnt[kBlock / 64];
-    __shared__ short order[kBlock];
     __shared__ unsigned arrived;  // waves done, for the block partials
 
     // grid (tiles, groups of RB right-hand sides); the XCD remap runs over the linear
@@ -94,7 +98,10 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     const int64_t ns = A.ns;
     const int tid = threadIdx.x;
-    if (tid == 0) arrived = 0u;  // ordered before use by the barriers below
+    if (tid == 0) arrived = 0u;  // ordered before use by the barrier below
+    // this lane's node: t in the tile's wave-cube numbering (homogeneous nodes by phase,
+    // then mixed ones by their phases; k_tile_order)
+    const int t = A.perm[(int64_t)bid * kSB + tid];
     // Interior tiles index without wrapping; boundary tiles wrap each coordinate.
     const bool interior = ox >= 1 && oy >= 1 && oz >= 1 && ox + TX + 1 <= nx && oy + TY + 1 <= ny && oz + TZ + 1 <= nz;
     const int corner = (oz - 1) * nxy + (oy - 1) * nx + (ox - 1);
@@ -133,12 +140,12 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             };
             if (interior) {
 #pragma unroll
-                for (int it = 0; it < (HN + kBlock - 1) / kBlock; it++) {
-                    const int q = tid + it * kBlock;
+                for (int it = 0; it < (HN + kSB - 1) / kSB; it++) {
+                    const int q = tid + it * kSB;
                     if (q < HN) put(q, corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX);
                 }
             } else {
-                for (int q = tid; q < HN; q += kBlock)
+                for (int q = tid; q < HN; q += kSB)
                     put(q, pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
                                nxy * pmod(oz + q / (HX * HY) - 1, nz));
             }
@@ -146,15 +153,15 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             const T *x0 = x + o, *x1 = x0 + ns, *x2 = x1 + ns;
             if (interior) {  // unrolled: all loads of a thread in flight together
 #pragma unroll
-                for (int it = 0; it < (HN + kBlock - 1) / kBlock; it++) {
-                    const int q = tid + it * kBlock;
+                for (int it = 0; it < (HN + kSB - 1) / kSB; it++) {
+                    const int q = tid + it * kSB;
                     if (q < HN) {
                         const int g = corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX;
                         hsr[q] = x0[g]; hsr[HN + q] = x1[g]; hsr[2 * HN + q] = x2[g];
                     }
                 }
             } else {
-                for (int q = tid; q < HN; q += kBlock) {
+                for (int q = tid; q < HN; q += kSB) {
                     const int g = pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
                                   nxy * pmod(oz + q / (HX * HY) - 1, nz);
                     hsr[q] = x0[g]; hsr[HN + q] = x1[g]; hsr[2 * HN + q] = x2[g];
@@ -166,12 +173,12 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // the voxel phases first: their loads overlap the state loads of the early exit
     if (interior) {
 #pragma unroll
-        for (int it = 0; it < (VN + kBlock - 1) / kBlock; it++) {
-            const int q = tid + it * kBlock;
+        for (int it = 0; it < (VN + kSB - 1) / kSB; it++) {
+            const int q = tid + it * kSB;
             if (q < VN) ps[q] = pix[corner + (q / (VX * VY)) * nxy + ((q / VX) % VY) * nx + q % VX];
         }
     } else {
-        for (int q = tid; q < VN; q += kBlock)
+        for (int q = tid; q < VN; q += kSB)
             ps[q] = pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
                         nxy * pmod(oz + q / (VX * VY) - 1, nz)];
     }
@@ -183,28 +190,8 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
         if ((live >> rb) & 1u) stage(rb);
     __syncthreads();
 
-    // Stable compaction: homogeneous nodes first, then mixed ones.
-    const int w = tid >> 6, l = tid & 63;
-    {
-        const int lx = (w & 1) * 4 + (l & 3), ly = (w >> 1) * 4 + ((l >> 2) & 3), lz = l >> 4;
-        const int *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
-        const int p0 = pc[0];
-        bool hom = true;
-#pragma unroll
-        for (int a = 1; a < 8; a++) hom &= pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))] == p0;
-        const unsigned long long bh = __ballot(hom);
-        if (l == 0) cnt[w] = __popcll(bh);
-        __syncthreads();
-        int nh = 0, before = 0;
-#pragma unroll
-        for (int q = 0; q < kBlock / 64; q++) { nh += cnt[q]; before += q < w ? cnt[q] : 0; }
-        const int rh = __popcll(bh & ((1ull << l) - 1ull));
-        order[hom ? before + rh : nh + (w * 64 - before) + (l - rh)] = (short)tid;
-        __syncthreads();
-    }
-    const int t = order[tid];
     const int tw = t >> 6, tl = t & 63;
-    const int lx = (tw & 1) * 4 + (tl & 3), ly = (tw >> 1) * 4 + ((tl >> 2) & 3), lz = tl >> 4;
+    const int lx = wave_x(tw) + (tl & 3), ly = wave_y(tw) + ((tl >> 2) & 3), lz = wave_z(tw) + (tl >> 4);
     const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
     const bool active = gx < nx && gy < ny && gz < nz;
     const int m = gx + nx * gy + nxy * gz;
@@ -362,7 +349,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             slot[rb] = r < nrhs ? (int64_t)r * ntile + bid : -1;
             if constexpr (MODE == kEnergy) slot[RB + rb] = r < nrhs ? (int64_t)(nrhs + r) * ntile + bid : -1;
         }
-        write_partials_nowait(part, A.partials, slot, &arrived);
+        write_partials_nowait<RB * NP, kSB>(part, A.partials, slot, &arrived);
     }
 }
 #undef E3X_NB
@@ -370,9 +357,56 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 // At least WPE waves per SIMD (six: the 80 VGPRs of the FP64 body, enough to cover the
 // scalar-cache latency of the coefficients; eight for the FP32 bodies, which fit in 64).
 template <typename T, int MODE, int RB, int WPE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_stencil(StencilArgs<T> A)
+__global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_stencil(StencilArgs<T> A)
 {
     stencil_body<T, MODE, RB>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold, A.hnew, A.u);
+}
+
+// Lane order of each stencil tile (Solver::perm), from the phases alone: one block per
+// tile, thread t = node t of the tile's wave-cube numbering.  Key: homogeneous nodes
+// first, by phase; then mixed nodes by (smallest, largest, octant-0 phase); nodes
+// outside the grid last; ties by t.  A bitonic sort of the 256 keys in LDS.
+__global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int ntx, int nty, const uint16_t *pix,
+                                                  uint8_t *perm)
+{
+    __shared__ int ps[VN];
+    __shared__ unsigned long long key[kSB];
+    const int bid = blockIdx.x, t = threadIdx.x;
+    const int tx = bid % ntx, ty = (bid / ntx) % nty, tz = bid / (ntx * nty);
+    const int ox = tx * TX, oy = ty * TY, oz = tz * TZ, nxy = nx * ny;
+    for (int q = t; q < VN; q += kSB)
+        ps[q] = pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
+                    nxy * pmod(oz + q / (VX * VY) - 1, nz)];
+    __syncthreads();
+    const int tw = t >> 6, tl = t & 63;
+    const int lx = wave_x(tw) + (tl & 3), ly = wave_y(tw) + ((tl >> 2) & 3), lz = wave_z(tw) + (tl >> 4);
+    const int *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
+    unsigned pmin = 0xFFFFu, pmax = 0u;
+    const unsigned p0 = (unsigned)pc[0];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const unsigned pa = (unsigned)pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
+        pmin = min(pmin, pa);
+        pmax = max(pmax, pa);
+    }
+    const bool active = ox + lx < nx && oy + ly < ny && oz + lz < nz;
+    const unsigned long long cls = !active ? 2ull : (pmin == pmax ? 0ull : 1ull);
+    key[t] = (cls << 62) | ((unsigned long long)pmin << 44) | ((unsigned long long)pmax << 28) |
+             ((unsigned long long)p0 << 12) | (unsigned long long)t;
+    __syncthreads();
+    for (int k = 2; k <= kSB; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int i = t, p = t ^ j;
+            if (p > i) {
+                const unsigned long long a = key[i], b = key[p];
+                if (((i & k) == 0) == (a > b)) {
+                    key[i] = b;
+                    key[p] = a;
+                }
+            }
+            __syncthreads();
+        }
+    perm[(int64_t)bid * kSB + t] = (uint8_t)(key[t] & 0xFFull);
 }
 
 template <int NR>
@@ -394,7 +428,7 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
     StencilArgs<T> A{};
     A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.nrhs = s.nrhs;
     A.ntx = (s.nx + TX - 1) / TX; A.nty = (s.ny + TY - 1) / TY;
-    A.ns = s.ns; A.pix = s.pix; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
+    A.ns = s.ns; A.pix = s.pix; A.perm = s.perm; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
     A.partials = s.partials; A.st = s.st;
     return A;
 }
@@ -406,7 +440,7 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
 template <typename T, int MODE>
 void dispatch_stencil(const Solver &s, const StencilArgs<T> &A)
 {
-    const dim3 grid((unsigned)stencil_blocks(s), (unsigned)s.nrhs), block(kBlock);
+    const dim3 grid((unsigned)stencil_blocks(s), (unsigned)s.nrhs), block(kSB);
     if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_stencil<T, MODE, 1, 8>), grid, block, 0, s.stream, A);
     else hipLaunchKernelGGL((k_stencil<T, MODE, 1, 6>), grid, block, 0, s.stream, A);
 }
@@ -484,6 +518,13 @@ hipError_t launch_apply_only(Solver &s, bool mixed)
         double *hb[2] = {s.h_alt, s.h};
         apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, 2, true);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_order(Solver &s)
+{
+    hipLaunchKernelGGL(k_tile_order, dim3((unsigned)stencil_blocks(s)), dim3(kSB), 0, s.stream, s.nx, s.ny, s.nz,
+                       (s.nx + TX - 1) / TX, (s.ny + TY - 1) / TY, (const uint16_t *)s.pix, s.perm);
     return hipGetLastError();
 }
 
