@@ -272,7 +272,7 @@ def main():
     traffic = measured_traffic("traffic_fp64_bytes_per_launch") if default_workload else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel="k_stencil<double,1,kApplyFused> (h = z + gamma h_prev, u -= lambda h_prev, "
+                    kernel="k_stencil<double, kApplyFused> (h = z + gamma h_prev, u -= lambda h_prev, "
                            "Ah = K h, h.Ah partials)",
                     avg_launch_us=round(fp64["apply_ms"] * 1e3, 2),
                     algorithmic_bytes_per_launch=alg_bytes)
