@@ -82,38 +82,22 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *partials
     }
 }
 
-// Block partials at partials[slot[q]] (none where slot[q] < 0), without a block barrier: each wave leaves its sum
-// in LDS and counts itself in; the last wave to arrive adds the four sums in wave order
-// (a fixed order, whichever wave is last) and stores the partial.  A finished wave
-// leaves at once instead of waiting for the slowest wave of its block.  `arrived` is a
-// block-shared counter set to 0 before a __syncthreads() that precedes this call; every
-// lane of every wave calls it (all lanes active).
-template <int NV, int NT = kBlock>
-__device__ __forceinline__ void write_partials_nowait(double (&v)[NV], double *partials, const int64_t (&slot)[NV],
-                                                      unsigned *arrived)
+// True in every lane of the last wave of its block to call it, false in the others,
+// without a block barrier: a wave that is done leaves at once instead of waiting for the
+// slowest wave of its block, and the last one finds in LDS everything the others wrote
+// there before their call.  `arrived` is a block-shared counter set to 0 before a
+// __syncthreads() that precedes the call; every wave calls it once, all lanes active.
+template <int NT>
+__device__ __forceinline__ bool last_wave(unsigned *arrived)
 {
-    constexpr int NW = NT / 64;
-    __shared__ double sh[NV][NW];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    double t[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) t[q] = wave_sum(v[q]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     unsigned prev = 0;
-    if (l == 63) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) sh[q][w] = t[q];
+    if ((threadIdx.x & 63) == 63)
         prev = __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
     prev = __builtin_amdgcn_readlane(prev, 63);
-    if (prev == NW - 1 && l == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double tot = sh[q][0];
-#pragma unroll
-            for (int ww = 1; ww < NW; ww++) tot += sh[q][ww];
-            if (slot[q] >= 0) partials[slot[q]] = tot;
-        }
-    }
+    if (prev != NT / 64 - 1) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
 }
 
 // Threads of the single-block finalize kernels.

@@ -53,6 +53,29 @@ __device__ __forceinline__ int wave_x(int w) { return (w % (TX / 4)) * 4; }
 __device__ __forceinline__ int wave_y(int w) { return ((w / (TX / 4)) % (TY / 4)) * 4; }
 __device__ __forceinline__ int wave_z(int w) { return (w / ((TX / 4) * (TY / 4))) * 4; }
 
+// Halo positions (hx, hy, hz), 0 <= h < H: thread n stages the tile's own node n (x
+// fastest, so its stores of the node go out row by row) and the shell positions n and
+// n + kSB of the kShell halo nodes around the tile (the two z faces, then per inner
+// plane its y = 0 and y = HY-1 rows and its x = 0 and x = HX-1 ends).
+struct HPos {
+    int x, y, z;
+};
+constexpr int kShell = HN - kSB;
+__device__ __forceinline__ HPos inner_pos(int n) { return {1 + n % TX, 1 + (n / TX) % TY, 1 + n / (TX * TY)}; }
+__device__ __forceinline__ HPos shell_pos(int s)
+{
+    constexpr int plane = HX * HY, ring = 2 * HX + 2 * (HY - 2);
+    if (s < 2 * plane) {
+        const int w = s % plane;
+        return {w % HX, w / HX, s < plane ? 0 : HZ - 1};
+    }
+    const int r = s - 2 * plane, w = r % ring, hz = 1 + r / ring;
+    if (w < 2 * HX) return {w % HX, w < HX ? 0 : HY - 1, hz};
+    const int v = w - 2 * HX;
+    return {(v & 1) * (HX - 1), 1 + (v >> 1), hz};
+}
+__device__ __forceinline__ int hpos_index(HPos p) { return p.z * (HX * HY) + p.y * HX + p.x; }
+
 // The phase tables are read-only while a stencil kernel runs.  Reading them through the
 // constant address space makes every wave-uniform coefficient load a scalar load
 // whatever the kernel stores elsewhere (through the global address space the compiler
@@ -68,11 +91,12 @@ __device__ __forceinline__ ctab<T> *as_table(const T *p) { return (ctab<T> *)p; 
 // The body takes the arrays as restrict parameters: the output stores then provably
 // leave the tables untouched, and coefficient loads after a store stay scalar.
 //
-// RB right-hand sides per block (grid: tiles x ceil(nrhs / RB)): the block stages the RB
-// halos, classifies its nodes once and applies every scalar coefficient it loads to all
-// RB of them.  A right-hand side past nrhs, or one that has stopped, is computed from
-// whatever its LDS slots hold and not stored.
-template <typename T, int MODE, int RB>
+// One right-hand side per block (grid: tiles x nrhs).  Every global load of the
+// staging (phases, this tile's lane order, the halo and, fused, u at the tile's own
+// nodes) is issued at the top, before the block learns from the solver state whether
+// its right-hand side still runs, so that the state's latency and the loads' overlap;
+// a stopped right-hand side drops what it loaded and leaves.
+template <typename T, int MODE>
 __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint16_t *__restrict__ pix,
                                              const T *__restrict__ dk, const T *__restrict__ sten,
                                              const T *__restrict__ x, T *__restrict__ y,
@@ -82,119 +106,112 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 {
     constexpr bool FUSED = MODE == kApplyFused;
     constexpr bool HAS_ST = MODE == kApply || MODE == kApplyFused || MODE == kResid;
-    constexpr int NP = MODE == kEnergy ? 2 : 1;  // partial sums per right-hand side
-    __shared__ T hs[RB * 3 * HN];
+    __shared__ T hs[3 * HN];
+    __shared__ T as[3 * kSB];  // Ah in x-fastest tile order
     __shared__ int ps[VN];
-    __shared__ unsigned arrived;  // waves done, for the block partials
+    __shared__ unsigned arrived;  // waves done (last_wave)
 
-    // grid (tiles, groups of RB right-hand sides); the XCD remap runs over the linear
-    // block id, whose remainder mod 8 is the XCD, and (tile, group) come from the
-    // remapped id
-    const int ntile = gridDim.x, ngroup = gridDim.y, nrhs = A.nrhs;
-    const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * ngroup);
-    const int grp = lin / ntile, bid = lin % ntile;
+    // grid (tiles, right-hand sides); the XCD remap runs over the linear block id, whose
+    // remainder mod 8 is the XCD, and (tile, right-hand side) come from the remapped id
+    const int ntile = gridDim.x, nrhs = A.nrhs;
+    const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+    const int r = lin / ntile, bid = lin % ntile;
     const int tx = bid % A.ntx, ty = (bid / A.ntx) % A.nty, tz = bid / (A.ntx * A.nty);
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
-    const int64_t ns = A.ns;
+    const int64_t ns = A.ns, o = (int64_t)(3 * r) * ns;
     const int tid = threadIdx.x;
     if (tid == 0) arrived = 0u;  // ordered before use by the barrier below
-    // this lane's node: t in the tile's wave-cube numbering (homogeneous nodes by phase,
-    // then mixed ones by their phases; k_tile_order)
-    const int t = A.perm[(int64_t)bid * kSB + tid];
+    // the solver state first, so that its latency overlaps the loads below (r < nrhs:
+    // the grid has nrhs rows)
+    int stop = 0, rst = 0;
+    double gam = 0.0, lam = 0.0;
+    if constexpr (HAS_ST) stop = A.st[r].stop_iter;
+    if constexpr (FUSED) {
+        gam = A.st[r].gamma;
+        lam = A.st[r].lambda;
+        rst = A.st[r].reset;
+    }
     // Interior tiles index without wrapping; boundary tiles wrap each coordinate.
     const bool interior = ox >= 1 && oy >= 1 && oz >= 1 && ox + TX + 1 <= nx && oy + TY + 1 <= ny && oz + TZ + 1 <= nz;
     const int corner = (oz - 1) * nxy + (oy - 1) * nx + (ox - 1);
-
-    unsigned live = 0u;  // bit rb: right-hand side grp*RB + rb exists and still runs
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
-        const int r = grp * RB + rb;
-        if (r < nrhs && (!HAS_ST || A.st[r].stop_iter == 0)) live |= 1u << rb;
-    }
-    auto stage = [&](int rb) {
-        const int r = grp * RB + rb;
-        const int64_t o = (int64_t)(3 * r) * ns;
-        T *hsr = hs + rb * 3 * HN;
-        if constexpr (FUSED) {
-            // this iteration's direction, h = z + gamma h_prev or h = z on a reset
-            // (f90:1961-1996); the block's own nodes also store it and take the previous
-            // step into u, u -= lambda h_prev (f90:1914-1918)
-            const double gam = A.st[r].gamma, lam = A.st[r].lambda;
-            const bool run = (live >> rb) & 1u, rst = A.st[r].reset != 0, dou = A.fuse_u != 0;
-            auto put = [&](int q, int g) {
-                const int hx = q % HX, hy = (q / HX) % HY, hz = q / (HX * HY);
-                const bool own = run && hx >= 1 && hx <= TX && hy >= 1 && hy <= TY && hz >= 1 && hz <= TZ &&
-                                 ox + hx - 1 < nx && oy + hy - 1 < ny && oz + hz - 1 < nz;
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const int64_t i = o + c * ns + g;
-                    const T zc = zv[i], hc = hold[i];
-                    const T hv = rst ? zc : (T)((double)zc + gam * (double)hc);
-                    hsr[c * HN + q] = hv;
-                    if (own) {
-                        hnew[i] = hv;
-                        if (dou) u[i] = u[i] - lam * (double)hc;
-                    }
-                }
-            };
-            if (interior) {
-#pragma unroll
-                for (int it = 0; it < (HN + kSB - 1) / kSB; it++) {
-                    const int q = tid + it * kSB;
-                    if (q < HN) put(q, corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX);
-                }
-            } else {
-                for (int q = tid; q < HN; q += kSB)
-                    put(q, pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
-                               nxy * pmod(oz + q / (HX * HY) - 1, nz));
-            }
-        } else {
-            const T *x0 = x + o, *x1 = x0 + ns, *x2 = x1 + ns;
-            if (interior) {  // unrolled: all loads of a thread in flight together
-#pragma unroll
-                for (int it = 0; it < (HN + kSB - 1) / kSB; it++) {
-                    const int q = tid + it * kSB;
-                    if (q < HN) {
-                        const int g = corner + (q / (HX * HY)) * nxy + ((q / HX) % HY) * nx + q % HX;
-                        hsr[q] = x0[g]; hsr[HN + q] = x1[g]; hsr[2 * HN + q] = x2[g];
-                    }
-                }
-            } else {
-                for (int q = tid; q < HN; q += kSB) {
-                    const int g = pmod(ox + q % HX - 1, nx) + nx * pmod(oy + (q / HX) % HY - 1, ny) +
-                                  nxy * pmod(oz + q / (HX * HY) - 1, nz);
-                    hsr[q] = x0[g]; hsr[HN + q] = x1[g]; hsr[2 * HN + q] = x2[g];
-                }
-            }
-        }
+    auto gpos = [&](HPos p) -> int {
+        return interior ? corner + p.z * nxy + p.y * nx + p.x
+                        : pmod(ox + p.x - 1, nx) + nx * pmod(oy + p.y - 1, ny) + nxy * pmod(oz + p.z - 1, nz);
     };
 
-    // the voxel phases first: their loads overlap the state loads of the early exit
-    if (interior) {
+    // --- loads -------------------------------------------------------------------
+    // voxel phases of the 9x9x5 voxel halo (positions tid and tid + kSB)
+    int pvl[2];
 #pragma unroll
-        for (int it = 0; it < (VN + kSB - 1) / kSB; it++) {
-            const int q = tid + it * kSB;
-            if (q < VN) ps[q] = pix[corner + (q / (VX * VY)) * nxy + ((q / VX) % VY) * nx + q % VX];
+    for (int k = 0; k < 2; k++) {
+        const int q = min(tid + k * kSB, VN - 1);
+        const HPos p = {q % VX, (q / VX) % VY, q / (VX * VY)};
+        pvl[k] = pix[gpos(p)];
+    }
+    // this lane's node: t in the tile's wave-cube numbering (homogeneous nodes by phase,
+    // then mixed ones by their phases; k_tile_order)
+    const int t = A.perm[(int64_t)bid * kSB + tid];
+    // halo: the tile's own node tid and shell positions tid, tid + kSB
+    const HPos pin = inner_pos(tid), sh0 = shell_pos(tid), sh1 = shell_pos(tid + kSB);
+    const bool has1 = tid + kSB < kShell;
+    const int64_t gi = o + gpos(pin), g0 = o + gpos(sh0), g1 = o + gpos(has1 ? sh1 : sh0);
+    const bool ingrid = ox + pin.x - 1 < nx && oy + pin.y - 1 < ny && oz + pin.z - 1 < nz;
+    const T *src = FUSED ? zv : x;
+    T ai[3], a0[3], a1[3];  // z (fused) or x
+    T bi[3], b0[3], b1[3];  // h_prev (fused)
+    double ui[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        ai[c] = src[gi + c * ns]; a0[c] = src[g0 + c * ns]; a1[c] = src[g1 + c * ns];
+        if constexpr (FUSED) {
+            bi[c] = hold[gi + c * ns]; b0[c] = hold[g0 + c * ns]; b1[c] = hold[g1 + c * ns];
+        }
+    }
+    if constexpr (FUSED) {
+        if (ingrid) {  // whether u is updated (fuse_u) is known later: a scalar wait here would hold these back
+#pragma unroll
+            for (int c = 0; c < 3; c++) ui[c] = u[gi + c * ns];
+        }
+    }
+
+    // --- stage ---------------------------------------------------------------------
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (tid + k * kSB < VN) ps[tid + k * kSB] = pvl[k];
+    if (stop != 0) return;  // uniform over the block
+    const int q = hpos_index(pin), q0 = hpos_index(sh0), q1 = hpos_index(sh1);
+    if constexpr (FUSED) {
+        // this iteration's direction, h = z + gamma h_prev or h = z on a reset
+        // (f90:1961-1996); the tile's own nodes also store it and take the previous step
+        // into u, u -= lambda h_prev (f90:1914-1918)
+        const bool dou = A.fuse_u != 0;
+        auto dir = [&](T zc, T hc) -> T { return rst != 0 ? zc : (T)((double)zc + gam * (double)hc); };
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const T hv = dir(ai[c], bi[c]);
+            hs[c * HN + q] = hv;
+            hs[c * HN + q0] = dir(a0[c], b0[c]);
+            if (has1) hs[c * HN + q1] = dir(a1[c], b1[c]);
+            if (ingrid) {
+                hnew[gi + c * ns] = hv;
+                if (dou) u[gi + c * ns] = ui[c] - lam * (double)bi[c];
+            }
         }
     } else {
-        for (int q = tid; q < VN; q += kSB)
-            ps[q] = pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
-                        nxy * pmod(oz + q / (VX * VY) - 1, nz)];
-    }
-    // One staging site only: with a second one (staging before this exit) the compiler
-    // no longer proves the tables unclobbered and loads the coefficients per lane.
-    if (!live) return;  // uniform over the block
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-        if ((live >> rb) & 1u) stage(rb);
+        for (int c = 0; c < 3; c++) {
+            hs[c * HN + q] = ai[c];
+            hs[c * HN + q0] = a0[c];
+            if (has1) hs[c * HN + q1] = a1[c];
+        }
+    }
     __syncthreads();
 
+    // --- apply ---------------------------------------------------------------------
     const int tw = t >> 6, tl = t & 63;
     const int lx = wave_x(tw) + (tl & 3), ly = wave_y(tw) + ((tl >> 2) & 3), lz = wave_z(tw) + (tl >> 4);
-    const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
-    const bool active = gx < nx && gy < ny && gz < nz;
-    const int m = gx + nx * gy + nxy * gz;
+    const bool active = ox + lx < nx && oy + ly < ny && oz + lz < nz;
     const int *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
     int pv[8];
 #pragma unroll
@@ -203,11 +220,9 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 #pragma unroll
     for (int a = 1; a < 8; a++) homog &= pv[a] == pv[0];
     const bool hlane = active && homog, mlane = active && !homog;
-    const T *hr = hs + (lz + 1) * (HX * HY) + (ly + 1) * HX + (lx + 1);  // rhs rb: hr + rb * 3 * HN
+    const T *hr = hs + (lz + 1) * (HX * HY) + (ly + 1) * HX + (lx + 1);
 
-    T acc[RB][3];
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++) acc[rb][0] = acc[rb][1] = acc[rb][2] = T(0);
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
     if (__ballot(hlane)) {
         bool pend = hlane;
         while (true) {
@@ -215,44 +230,29 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             if (pm == 0ull) break;
             const int P = __builtin_amdgcn_readlane(pv[0], __builtin_ctzll(pm));
             ctab<T> *S = as_table(sten) + (int64_t)P * kStenHalf;
-            T tt[RB][3];
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) tt[rb][0] = tt[rb][1] = tt[rb][2] = T(0);
+            T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
             for (int k = 0; k < 13; k++) {
 #pragma unroll
                 for (int n = 0; n < 3; n++) {  // neighbour +delta
                     const T s0 = S[k * 9 + 0 * 3 + n], s1 = S[k * 9 + 1 * 3 + n], s2 = S[k * 9 + 2 * 3 + n];
-#pragma unroll
-                    for (int rb = 0; rb < RB; rb++) {
-                        const T hp = E3X_NB(hr + rb * 3 * HN, n, 14 + k);
-                        tt[rb][0] += s0 * hp; tt[rb][1] += s1 * hp; tt[rb][2] += s2 * hp;
-                    }
+                    const T hp = E3X_NB(hr, n, 14 + k);
+                    t0 += s0 * hp; t1 += s1 * hp; t2 += s2 * hp;
                 }
 #pragma unroll
                 for (int n = 0; n < 3; n++) {  // neighbour -delta: S[-d][j][n] = S[d][n][j]
                     const T s0 = S[k * 9 + n * 3 + 0], s1 = S[k * 9 + n * 3 + 1], s2 = S[k * 9 + n * 3 + 2];
-#pragma unroll
-                    for (int rb = 0; rb < RB; rb++) {
-                        const T hq = E3X_NB(hr + rb * 3 * HN, n, 12 - k);
-                        tt[rb][0] += s0 * hq; tt[rb][1] += s1 * hq; tt[rb][2] += s2 * hq;
-                    }
+                    const T hq = E3X_NB(hr, n, 12 - k);
+                    t0 += s0 * hq; t1 += s1 * hq; t2 += s2 * hq;
                 }
             }
             ctab<T> *Cc = S + 117;  // centre: 00 11 22 01 02 12
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                const T *h0 = hr + rb * 3 * HN;
-                const T c0 = h0[0], c1 = h0[HN], c2 = h0[2 * HN];
-                tt[rb][0] += Cc[0] * c0; tt[rb][0] += Cc[3] * c1; tt[rb][0] += Cc[4] * c2;
-                tt[rb][1] += Cc[3] * c0; tt[rb][1] += Cc[1] * c1; tt[rb][1] += Cc[5] * c2;
-                tt[rb][2] += Cc[4] * c0; tt[rb][2] += Cc[5] * c1; tt[rb][2] += Cc[2] * c2;
-            }
+            const T c0 = hr[0], c1 = hr[HN], c2 = hr[2 * HN];
+            t0 += Cc[0] * c0; t0 += Cc[3] * c1; t0 += Cc[4] * c2;
+            t1 += Cc[3] * c0; t1 += Cc[1] * c1; t1 += Cc[5] * c2;
+            t2 += Cc[4] * c0; t2 += Cc[5] * c1; t2 += Cc[2] * c2;
             if (pend && pv[0] == P) {
-#pragma unroll
-                for (int rb = 0; rb < RB; rb++) {
-                    acc[rb][0] = tt[rb][0]; acc[rb][1] = tt[rb][1]; acc[rb][2] = tt[rb][2];
-                }
+                acc0 = t0; acc1 = t1; acc2 = t2;
                 pend = false;
             }
         }
@@ -262,14 +262,11 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
         for (int a = 0; a < 8; a++) {
             // voxel a of the node spans nodes (PX(b)-PX(a), PY(b)-PY(a), PZ(b)-PZ(a)), b = 0..7
             const T *hb = hr - (PZ(a) * (HX * HY) + PY(a) * HX + PX(a));
-            T hv[RB][24];
+            T hv[24];
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++)
+            for (int b = 0; b < 8; b++)
 #pragma unroll
-                for (int b = 0; b < 8; b++)
-#pragma unroll
-                    for (int n = 0; n < 3; n++)
-                        hv[rb][b * 3 + n] = hb[rb * 3 * HN + n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
+                for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
             const int pa = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
             bool pend = mlane;
             while (true) {
@@ -277,89 +274,89 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
                 if (pm == 0ull) break;
                 const int P = __builtin_amdgcn_readlane(pa, __builtin_ctzll(pm));
                 ctab<T> *D = as_table(dk) + (int64_t)P * 576 + a * 72;
-                T tt[RB][3];
-#pragma unroll
-                for (int rb = 0; rb < RB; rb++) tt[rb][0] = tt[rb][1] = tt[rb][2] = T(0);
+                T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
                 for (int j = 0; j < 24; j++) {
                     const T d0 = D[j], d1 = D[24 + j], d2 = D[48 + j];
-#pragma unroll
-                    for (int rb = 0; rb < RB; rb++) {
-                        tt[rb][0] += d0 * hv[rb][j]; tt[rb][1] += d1 * hv[rb][j]; tt[rb][2] += d2 * hv[rb][j];
-                    }
+                    t0 += d0 * hv[j]; t1 += d1 * hv[j]; t2 += d2 * hv[j];
                 }
                 if (pend && pa == P) {
-#pragma unroll
-                    for (int rb = 0; rb < RB; rb++) {
-                        acc[rb][0] += tt[rb][0]; acc[rb][1] += tt[rb][1]; acc[rb][2] += tt[rb][2];
-                    }
+                    acc0 += t0; acc1 += t1; acc2 += t2;
                     pend = false;
                 }
             }
         }
     }
 
-    double part[RB * NP];
+    // --- store ---------------------------------------------------------------------
+    // Ah into LDS at each lane's node in x-fastest tile order; the last wave of the block
+    // to get here stores the tile row by row (the lanes hold their nodes in phase order)
+    // and forms the partial sums, each lane over its four nodes in turn, then the wave.
+    {
+        const int nn = lx + TX * (ly + TY * lz);
+        as[nn] = acc0; as[kSB + nn] = acc1; as[2 * kSB + nn] = acc2;
+    }
+    if (!last_wave<kSB>(&arrived)) return;
+    int l = tid & 63;
+    asm volatile("" : "+v"(l));  // keeps the tile addressing below from being hoisted above the FMA loops (spills)
+    double p0 = 0.0, p1 = 0.0;  // h.Ah (or utot), gg (kEnergy)
 #pragma unroll
-    for (int q = 0; q < RB * NP; q++) part[q] = 0.0;
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
-        if (!active || !((live >> rb) & 1u)) continue;
-        const int64_t ro = (int64_t)(3 * (grp * RB + rb)) * ns;  // this right-hand side's vectors
-        const T *h0 = hr + rb * 3 * HN;
+    for (int k = 0; k < kSB / 64; k++) {
+        const int n = k * 64 + l;
+        const int ex = n % TX, ey = (n / TX) % TY, ez = n / (TX * TY);
+        if (ox + ex >= nx || oy + ey >= ny || oz + ez >= nz) continue;
+        const int64_t m = o + (ox + ex) + nx * (oy + ey) + nxy * (oz + ez);
+        const T *h0 = hs + (ez + 1) * (HX * HY) + (ey + 1) * HX + (ex + 1);
+        const T *an = as + n;
         if constexpr (MODE == kApply || MODE == kApplyFused) {
             double pr = 0.0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                y[ro + j * ns + m] = acc[rb][j];
-                pr += (double)h0[j * HN] * (double)acc[rb][j];
+                const T aj = an[j * kSB];
+                y[m + j * ns] = aj;
+                pr += (double)h0[j * HN] * (double)aj;
             }
-            part[rb] = pr;
+            p0 += pr;
         } else if constexpr (MODE == kEnergy) {
             double e = 0.0, gsq = 0.0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const int64_t q = ro + j * ns + m;
-                const double uq = (double)h0[j * HN], bq = bvec[q], kx = (double)acc[rb][j];
+                const int64_t qq = m + j * ns;
+                const double uq = (double)h0[j * HN], bq = bvec[qq], kx = (double)an[j * kSB];
                 e = e + 0.5 * uq * kx + bq * uq;  // f90:1336
                 const double gq = kx + bq;         // f90:1337
-                y[q] = (T)gq;
+                y[qq] = (T)gq;
                 gsq += gq * gq;                     // f90:452
             }
-            part[rb] = e;
-            part[RB + rb] = gsq;
+            p0 += e;
+            p1 += gsq;
         } else if constexpr (MODE == kResid) {
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int64_t q = ro + j * ns + m;
-                y32[q] = (float)((double)acc[rb][j] + bvec[q]);
-            }
+            for (int j = 0; j < 3; j++) y32[m + j * ns] = (float)((double)an[j * kSB] + bvec[m + j * ns]);
         } else {
 #pragma unroll
-            for (int j = 0; j < 3; j++) y[ro + j * ns + m] = acc[rb][j];
+            for (int j = 0; j < 3; j++) y[m + j * ns] = an[j * kSB];
         }
     }
-    // partial slots [q * nrhs + r][tile]: h.Ah (or utot), then gg for kEnergy; none for a
-    // right-hand side past nrhs
+    // partial slots [q * nrhs + r][tile]: h.Ah (or utot), then gg for kEnergy
     if constexpr (MODE == kApply || MODE == kApplyFused || MODE == kEnergy) {
-        int64_t slot[RB * NP];
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-            const int r = grp * RB + rb;
-            slot[rb] = r < nrhs ? (int64_t)r * ntile + bid : -1;
-            if constexpr (MODE == kEnergy) slot[RB + rb] = r < nrhs ? (int64_t)(nrhs + r) * ntile + bid : -1;
+        const double t0 = wave_sum(p0);
+        double t1 = 0.0;
+        if constexpr (MODE == kEnergy) t1 = wave_sum(p1);
+        if (l == 63) {
+            A.partials[(int64_t)r * ntile + bid] = t0;
+            if constexpr (MODE == kEnergy) A.partials[(int64_t)(nrhs + r) * ntile + bid] = t1;
         }
-        write_partials_nowait<RB * NP, kSB>(part, A.partials, slot, &arrived);
     }
 }
 #undef E3X_NB
 
 // At least WPE waves per SIMD: six for the FP64 body (80 VGPRs; forced into 64 VGPRs
 // for eight waves it hoists fewer loads and ran 7 % slower), eight for the FP32 body.
-template <typename T, int MODE, int RB, int WPE>
+template <typename T, int MODE, int WPE>
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_stencil(StencilArgs<T> A)
 {
-    stencil_body<T, MODE, RB>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold, A.hnew, A.u);
+    stencil_body<T, MODE>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold, A.hnew, A.u);
 }
 
 // Lane order of each stencil tile (Solver::perm), from the phases alone: one block per
@@ -433,16 +430,16 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
     return A;
 }
 
-// One right-hand side per block (grid y = nrhs).  The body also serves RB > 1 (shared
-// staging of the phases, classification and coefficient loads), but at 200^3 two FP32
-// right-hand sides per block ran 2-4 % slower than one: the per-rhs FMA, LDS and
-// staging work dominates, and the second halo costs occupancy.
+// One right-hand side per block (grid y = nrhs).  (Two FP32 right-hand sides per block,
+// sharing the phase staging, classification and coefficient loads, ran 2-4 % slower at
+// 200^3: the per-rhs FMA, LDS and staging work dominates, and the second halo costs
+// occupancy.)
 template <typename T, int MODE>
 void dispatch_stencil(const Solver &s, const StencilArgs<T> &A)
 {
     const dim3 grid((unsigned)stencil_blocks(s), (unsigned)s.nrhs), block(kSB);
-    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_stencil<T, MODE, 1, 8>), grid, block, 0, s.stream, A);
-    else hipLaunchKernelGGL((k_stencil<T, MODE, 1, 6>), grid, block, 0, s.stream, A);
+    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_stencil<T, MODE, 8>), grid, block, 0, s.stream, A);
+    else hipLaunchKernelGGL((k_stencil<T, MODE, 6>), grid, block, 0, s.stream, A);
 }
 
 }  // namespace
