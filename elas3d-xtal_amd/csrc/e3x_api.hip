@@ -586,8 +586,10 @@ int e3x_get_field(e3x_solver *h, int which, int rhs, double *host)
                         : which == E3X_FIELD_H ? s.h : nullptr;
     if (!src) return fail(E3X_EINVAL, "unknown field");
     E3X_HIP(hipSetDevice(s.device));
-    E3X_HIP(hipMemcpyAsync(host, src + (int64_t)rhs * 3 * s.ns, sizeof(double) * 3 * (size_t)s.ns,
-                           hipMemcpyDeviceToHost, s.stream));
+    // the device holds [ns][3] per right-hand side; the caller gets u(ns, 3).  Ah is free
+    // outside a solve and serves as staging.
+    E3X_HIP(launch_nodes_to_planes(src + (int64_t)rhs * 3 * s.ns, s.ah, s.ns, s.stream));
+    E3X_HIP(hipMemcpyAsync(host, s.ah, sizeof(double) * 3 * (size_t)s.ns, hipMemcpyDeviceToHost, s.stream));
     E3X_HIP(hipStreamSynchronize(s.stream));
     return E3X_OK;
 }
@@ -602,8 +604,8 @@ int e3x_set_field(e3x_solver *h, int which, int rhs, const double *host)
                   : which == E3X_FIELD_H ? s.h : nullptr;
     if (!dst) return fail(E3X_EINVAL, "unknown field");
     E3X_HIP(hipSetDevice(s.device));
-    E3X_HIP(hipMemcpyAsync(dst + (int64_t)rhs * 3 * s.ns, host, sizeof(double) * 3 * (size_t)s.ns,
-                           hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(hipMemcpyAsync(s.ah, host, sizeof(double) * 3 * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(launch_planes_to_nodes(s.ah, dst + (int64_t)rhs * 3 * s.ns, s.ns, s.stream));
     E3X_HIP(hipStreamSynchronize(s.stream));
     return E3X_OK;
 }

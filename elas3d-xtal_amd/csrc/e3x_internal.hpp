@@ -60,9 +60,9 @@ struct Solver {
     // their phases, so that waves hold few distinct phases; built on upload
     uint8_t *perm = nullptr;                                   // [tiles][kSB]
 
-    // fields, [nrhs][3][ns]
+    // fields, [nrhs][ns][3] (the three components of a node side by side)
     double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;
-    double *minv = nullptr;                                    // [6][ns]
+    double *minv = nullptr;                                    // [ns][6]
     float *g32 = nullptr, *h32 = nullptr, *z32 = nullptr, *ah32 = nullptr, *minv32 = nullptr;
     // Second direction buffer: the direction of PCG iteration k lives in h (k odd) or
     // h_alt (k even), since the apply that forms it reads the previous one as halo.
@@ -124,6 +124,9 @@ hipError_t launch_stress(Solver &s);                          // results[0..12*n
 hipError_t launch_fullfield(Solver &s, int rhs, double *stress_dev, double *vm_dev);
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
 hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st);
+// one right-hand side's field: component planes [3][ns] <-> node-interleaved [ns][3]
+hipError_t launch_planes_to_nodes(const double *src, double *dst, int64_t ns, hipStream_t st);
+hipError_t launch_nodes_to_planes(const double *src, double *dst, int64_t ns, hipStream_t st);
 
 // e3x_api.hip: reuse a handle for another microstructure with nphase <= nphase_cap
 // phases (the serving loop's handle pool); the microstructure and tables must be

@@ -36,8 +36,8 @@ template <typename T>
 __device__ __forceinline__ void precond_apply(const T *minv, int64_t ns, int64_t m, double g0, double g1,
                                               double g2, double &z0, double &z1, double &z2)
 {
-    const double a00 = minv[m], a11 = minv[ns + m], a22 = minv[2 * ns + m];
-    const double a01 = minv[3 * ns + m], a02 = minv[4 * ns + m], a12 = minv[5 * ns + m];
+    const T *mi = minv + 6 * m;
+    const double a00 = mi[0], a11 = mi[1], a22 = mi[2], a01 = mi[3], a02 = mi[4], a12 = mi[5];
     z0 = a00 * g0 + a01 * g1 + a02 * g2;
     z1 = a01 * g0 + a11 * g1 + a12 * g2;
     z2 = a02 * g0 + a12 * g1 + a22 * g2;
@@ -54,13 +54,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs<T> A)
     for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            T *g = A.g + (int64_t)r * 3 * ns, *z = A.z + (int64_t)r * 3 * ns, *h = A.h + (int64_t)r * 3 * ns;
-            const double g0 = g[m], g1 = g[ns + m], g2 = g[2 * ns + m];
+            const int64_t o = ((int64_t)r * ns + m) * 3;
+            T *g = A.g + o, *z = A.z + o, *h = A.h + o;
+            const double g0 = g[0], g1 = g[1], g2 = g[2];
             double z0, z1, z2;
             precond_apply(A.minv, ns, m, g0, g1, g2, z0, z1, z2);
             const T t0 = (T)z0, t1 = (T)z1, t2 = (T)z2;
-            z[m] = t0; z[ns + m] = t1; z[2 * ns + m] = t2;
-            if (A.kkk == 1) { h[m] = t0; h[ns + m] = t1; h[2 * ns + m] = t2; }
+            z[0] = t0; z[1] = t1; z[2] = t2;
+            if (A.kkk == 1) { h[0] = t0; h[1] = t1; h[2] = t2; }
             part[r] += (double)t0 * g0 + (double)t1 * g1 + (double)t2 * g2;
         }
     }
@@ -136,23 +137,23 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
         double mi[6];
         if constexpr (DO_Z) {
 #pragma unroll
-            for (int q = 0; q < 6; q++) mi[q] = (double)minv[q * ns + m];
+            for (int q = 0; q < 6; q++) mi[q] = (double)minv[6 * m + q];
         }
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             if (!act[r]) continue;
-            const int64_t o = (int64_t)r * 3 * ns + m;
+            const int64_t o = ((int64_t)r * ns + m) * 3;
             if constexpr (DO_U) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) u[o + c * ns] = u[o + c * ns] - lam[r] * (double)hv[o + c * ns];
+                for (int c = 0; c < 3; c++) u[o + c] = u[o + c] - lam[r] * (double)hv[o + c];
             }
             double g[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                g[c] = (double)gv[o + c * ns];
+                g[c] = (double)gv[o + c];
                 if constexpr (UPDATE_G) {
-                    const T gn = (T)(g[c] - lam[r] * (double)ah[o + c * ns]);
-                    gv[o + c * ns] = gn;
+                    const T gn = (T)(g[c] - lam[r] * (double)ah[o + c]);
+                    gv[o + c] = gn;
                     g[c] = (double)gn;
                 }
             }
@@ -162,7 +163,7 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
                 const double z1 = mi[3] * g[0] + mi[1] * g[1] + mi[5] * g[2];
                 const double z2 = mi[4] * g[0] + mi[5] * g[1] + mi[2] * g[2];
                 const T t0 = (T)z0, t1 = (T)z1, t2 = (T)z2;
-                zv[o] = t0; zv[o + ns] = t1; zv[o + 2 * ns] = t2;
+                zv[o] = t0; zv[o + 1] = t1; zv[o + 2] = t2;
                 part[r] += (double)t0 * g[0] + (double)t1 * g[1] + (double)t2 * g[2];
             }
         }
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_finalize(PcgArgs<T> A, T *h_alt,
             const T *hk = (A.st[r].stop_iter & 1) ? A.h : h_alt;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const int64_t o = (int64_t)(3 * r + c) * ns + m;
+                const int64_t o = ((int64_t)r * ns + m) * 3 + c;
                 const T hv = hk[o];
                 if (mode[r] == 2) {
                     A.h[o] = hv;

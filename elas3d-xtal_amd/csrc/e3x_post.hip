@@ -26,7 +26,7 @@ __device__ void voxel_strain_stress(const PostArgs &A, int r, int64_t m, double 
     double uu[8][3];
     for (int a = 0; a < 8; a++) {
         const int64_t q = wrap(i + PX(a), A.nx) + (int64_t)A.nx * (wrap(j + PY(a), A.ny) + (int64_t)A.ny * wrap(k + PZ(a), A.nz));
-        for (int c = 0; c < 3; c++) uu[a][c] = u[c * A.ns + q];
+        for (int c = 0; c < 3; c++) uu[a][c] = u[3 * q + c];
     }
     if (i == A.nx - 1)
         for (int a = 0; a < 8; a++)
@@ -111,6 +111,26 @@ __global__ __launch_bounds__(kBlock) void k_to_f64(const float *s, double *d, in
     if (i < n) d[i] = (double)s[i];
 }
 
+// Between the reference's field layout u(ns, 3) (component planes) and the device's
+// node-interleaved [ns][3]; one thread per node.
+__global__ __launch_bounds__(kBlock) void k_planes_to_nodes(const double *s, double *d, int64_t ns)
+{
+    const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (m < ns) {
+        const double a = s[m], b = s[ns + m], c = s[2 * ns + m];
+        d[3 * m] = a; d[3 * m + 1] = b; d[3 * m + 2] = c;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_nodes_to_planes(const double *s, double *d, int64_t ns)
+{
+    const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (m < ns) {
+        const double a = s[3 * m], b = s[3 * m + 1], c = s[3 * m + 2];
+        d[m] = a; d[ns + m] = b; d[2 * ns + m] = c;
+    }
+}
+
 PostArgs post_args(const Solver &s)
 {
     PostArgs A{};
@@ -160,6 +180,20 @@ hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s
 hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_to_f64, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_planes_to_nodes(const double *src, double *dst, int64_t ns, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_planes_to_nodes, dim3((unsigned)((ns + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, src,
+                       dst, ns);
+    return hipGetLastError();
+}
+
+hipError_t launch_nodes_to_planes(const double *src, double *dst, int64_t ns, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_nodes_to_planes, dim3((unsigned)((ns + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, src,
+                       dst, ns);
     return hipGetLastError();
 }
 

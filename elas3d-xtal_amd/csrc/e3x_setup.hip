@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *p
                     acc[nn] += sv;
                 }
             }
-            for (int nn = 0; nn < 3; nn++) b[((int64_t)r * 3 + nn) * A.ns + m] = acc[nn];
+            for (int nn = 0; nn < 3; nn++) b[((int64_t)r * A.ns + m) * 3 + nn] = acc[nn];
             // constant of the voxel whose lower corner is m
             const int fx = i == A.nx - 1, fy = j == A.ny - 1, fz = k == A.nz - 1;
             if (fx | fy | fz) {
@@ -276,10 +276,10 @@ __global__ __launch_bounds__(kBlock) void k_affine(int nx, int ny, int nz, int n
     for (int r = 0; r < nrhs; r++) {
         const double *e = A.e + 6 * r;
         const double exx = e[0], eyy = e[1], ezz = e[2], exz = e[3], eyz = e[4], exy = e[5];
-        double *ur = u + (int64_t)r * 3 * ns;
-        ur[m] = x * exx + y * exy + z * exz;
-        ur[ns + m] = x * exy + y * eyy + z * eyz;
-        ur[2 * ns + m] = x * exz + y * eyz + z * ezz;
+        double *ur = u + ((int64_t)r * ns + m) * 3;
+        ur[0] = x * exx + y * exy + z * exz;
+        ur[1] = x * exy + y * eyy + z * eyz;
+        ur[2] = x * exz + y * eyz + z * ezz;
     }
 }
 
@@ -320,8 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int6
         o[3] = c12 * inv; o[4] = c13 * inv; o[5] = c23 * inv;
     }
     for (int q = 0; q < 6; q++) {
-        minv[(int64_t)q * ns + m] = o[q];
-        if (minv32) minv32[(int64_t)q * ns + m] = (float)o[q];
+        minv[6 * m + q] = o[q];
+        if (minv32) minv32[6 * m + q] = (float)o[q];
     }
 }
 

@@ -32,8 +32,8 @@ struct StencilArgs {
     const uint16_t *pix;
     const uint8_t *perm;   // lane order per tile (Solver::perm)
     const T *dk, *sten;
-    const T *x;            // [nrhs][3][ns]
-    T *y;                  // [nrhs][3][ns] (kApply, kEnergy, kPlain)
+    const T *x;            // [nrhs][ns][3]
+    T *y;                  // [nrhs][ns][3] (kApply, kEnergy, kPlain)
     const double *b;       // kEnergy, kResid
     float *y32;            // kResid output
     double *partials;
@@ -155,7 +155,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // halo: the tile's own node tid and shell positions tid, tid + kSB
     const HPos pin = inner_pos(tid), sh0 = shell_pos(tid), sh1 = shell_pos(tid + kSB);
     const bool has1 = tid + kSB < kShell;
-    const int64_t gi = o + gpos(pin), g0 = o + gpos(sh0), g1 = o + gpos(has1 ? sh1 : sh0);
+    const int64_t gi = o + 3 * (int64_t)gpos(pin), g0 = o + 3 * (int64_t)gpos(sh0), g1 = o + 3 * (int64_t)gpos(has1 ? sh1 : sh0);
     const bool ingrid = ox + pin.x - 1 < nx && oy + pin.y - 1 < ny && oz + pin.z - 1 < nz;
     const T *src = FUSED ? zv : x;
     T ai[3], a0[3], a1[3];  // z (fused) or x
@@ -163,15 +163,15 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     double ui[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        ai[c] = src[gi + c * ns]; a0[c] = src[g0 + c * ns]; a1[c] = src[g1 + c * ns];
+        ai[c] = src[gi + c]; a0[c] = src[g0 + c]; a1[c] = src[g1 + c];
         if constexpr (FUSED) {
-            bi[c] = hold[gi + c * ns]; b0[c] = hold[g0 + c * ns]; b1[c] = hold[g1 + c * ns];
+            bi[c] = hold[gi + c]; b0[c] = hold[g0 + c]; b1[c] = hold[g1 + c];
         }
     }
     if constexpr (FUSED) {
         if (ingrid) {  // whether u is updated (fuse_u) is known later: a scalar wait here would hold these back
 #pragma unroll
-            for (int c = 0; c < 3; c++) ui[c] = u[gi + c * ns];
+            for (int c = 0; c < 3; c++) ui[c] = u[gi + c];
         }
     }
 
@@ -194,8 +194,8 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             hs[c * HN + q0] = dir(a0[c], b0[c]);
             if (has1) hs[c * HN + q1] = dir(a1[c], b1[c]);
             if (ingrid) {
-                hnew[gi + c * ns] = hv;
-                if (dou) u[gi + c * ns] = ui[c] - lam * (double)bi[c];
+                hnew[gi + c] = hv;
+                if (dou) u[gi + c] = ui[c] - lam * (double)bi[c];
             }
         }
     } else {
@@ -305,7 +305,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
         const int n = k * 64 + l;
         const int ex = n % TX, ey = (n / TX) % TY, ez = n / (TX * TY);
         if (ox + ex >= nx || oy + ey >= ny || oz + ez >= nz) continue;
-        const int64_t m = o + (ox + ex) + nx * (oy + ey) + nxy * (oz + ez);
+        const int64_t m = o + 3 * (int64_t)((ox + ex) + nx * (oy + ey) + nxy * (oz + ez));
         const T *h0 = hs + (ez + 1) * (HX * HY) + (ey + 1) * HX + (ex + 1);
         const T *an = as + n;
         if constexpr (MODE == kApply || MODE == kApplyFused) {
@@ -313,7 +313,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const T aj = an[j * kSB];
-                y[m + j * ns] = aj;
+                y[m + j] = aj;
                 pr += (double)h0[j * HN] * (double)aj;
             }
             p0 += pr;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             double e = 0.0, gsq = 0.0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const int64_t qq = m + j * ns;
+                const int64_t qq = m + j;
                 const double uq = (double)h0[j * HN], bq = bvec[qq], kx = (double)an[j * kSB];
                 e = e + 0.5 * uq * kx + bq * uq;  // f90:1336
                 const double gq = kx + bq;         // f90:1337
@@ -332,10 +332,10 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             p1 += gsq;
         } else if constexpr (MODE == kResid) {
 #pragma unroll
-            for (int j = 0; j < 3; j++) y32[m + j * ns] = (float)((double)an[j * kSB] + bvec[m + j * ns]);
+            for (int j = 0; j < 3; j++) y32[m + j] = (float)((double)an[j * kSB] + bvec[m + j]);
         } else {
 #pragma unroll
-            for (int j = 0; j < 3; j++) y[m + j * ns] = an[j * kSB];
+            for (int j = 0; j < 3; j++) y[m + j] = an[j * kSB];
         }
     }
     // partial slots [q * nrhs + r][tile]: h.Ah (or utot), then gg for kEnergy
