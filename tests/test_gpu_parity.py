@@ -191,7 +191,8 @@ def test_divergence_reset_matches_oracle():
     assert _rel(u, ro["u"]) < 1e-6
 
 
-def test_hah_break_on_all_pore_grid():
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_hah_break_on_all_pore_grid(precision):
     shape = (4, 4, 4)
     with Elas3D(shape, 2) as s:
         s.ppixel(np.full(64, 2, np.int32), np.zeros(3))
@@ -200,7 +201,7 @@ def test_hah_break_on_all_pore_grid():
         u0 = s.get(L.FIELD_U)
         ut, gg0 = s.energy()
         assert gg0[0] == 0.0
-        d = s.dembx(gg0)
+        d = s.dembx(gg0, precision=precision)
         assert d["exit"][0] == L.PCG_HAH_BREAK and d["lstep"][0] == 1
         assert np.array_equal(s.get(L.FIELD_U), u0)
 
@@ -228,6 +229,37 @@ def test_maxiter_and_history_prefix(ldemb):
     assert _rel(h, r_o["h"]) < 1e-10
 
 
+@pytest.mark.parametrize("ldemb,replace_every", [(15, 0), (16, 0), (17, 0), (17, 8)])
+def test_mixed_steps_reach_u_once_in_order(ldemb, replace_every):
+    """The mixed form stopped by the iteration limit after an odd or an even number of
+    iterations (its u step of iteration k lands in the apply of iteration k+1, or in
+    the update of a replacement iteration, or after the loop), without and with
+    replacements every 8 iterations: the PCG correction u - u0 matches the FP64
+    oracle's after the same number of iterations to FP32 rounding; one step lost or
+    applied twice is off by more than ten times the tolerance (consecutive iterates
+    here differ by ~1e-4 of it)."""
+    shape = (12, 12, 12)
+    pix, rod = voronoi_case(shape, 5, seed=400)
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    r_o, _ = _oracle_pcg(shape, pix, rod, e6, ldemb=ldemb)
+    assert r_o["status"] == oc.OC_MAXITER
+    u0 = oc.affine(shape, e6)
+    with Elas3D(shape, 6) as s:
+        s.ppixel(pix, rod)
+        s.femat(C11, C12, C44, e6)
+        s.affine_guess()
+        _, gg0 = s.energy()
+        d = s.dembx(gg0, ldemb=ldemb, precision=L.PREC_MIXED, replace_every=replace_every)
+        u = s.get(L.FIELD_U)
+        h = s.get(L.FIELD_H)
+    assert d["exit"][0] == L.PCG_MAXITER and d["lstep"][0] == ldemb
+    err_u, err_h = _rel(u - u0, r_o["u"] - u0), _rel(h, r_o["h"])
+    print(f"ldemb {ldemb} replace {replace_every}: correction {err_u:.2e}, direction {err_h:.2e}")
+    assert err_u < 1e-5
+    assert err_h < 1e-3
+    assert _rel(d["history"][0], r_o["history"]) < 1e-4
+
+
 @pytest.mark.parametrize("replace_every", [0, 7, 1000])
 def test_mixed_precision_against_fp64_oracle(replace_every):
     """FP32 stencil/vectors/preconditioner with FP64 u and dots, true-residual
@@ -247,8 +279,8 @@ def test_mixed_precision_against_fp64_oracle(replace_every):
 
 @pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
 def test_multi_rhs_equals_single_solves(precision):
-    """Three right-hand sides in one PCG give each one's single solve (the mixed form's
-    kernels serve two per block, so the third shares its block with an unused slot)."""
+    """Three right-hand sides in one PCG give each one's single solve (they stop at
+    different iterations)."""
     shape = (12, 10, 8)
     pix, rod = voronoi_case(shape, 6, seed=500)
     strains = np.array([[oc.AML_F32, 0, 0, 0, 0, 0], [0, 0, 0, 0.5e-3, 0, 0], [0, 0.7e-3, 0, 0, 0, 0.2e-3]])

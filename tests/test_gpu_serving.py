@@ -67,6 +67,7 @@ def _handle_bytes(shape, mixed, ldemb=10000, phases=1024):
     """Device bytes the server books for a one-rhs handle (its admission estimate)."""
     ns = int(np.prod(shape))
     b = ns * 2 + 7 * 3 * ns * 8 + 6 * ns * 8
+    b += -(-shape[0] // 8) * -(-shape[1] // 8) * -(-shape[2] // 4) * 256   # stencil lane order
     b += phases * ((36 + 576 + 126 + 3) * 8 + (576 + 126) * 4)
     b += max(12 * 2048, 2 * ((ns + 255) // 256)) * 8 + (1 << 16) + ldemb * 8
     if mixed:
