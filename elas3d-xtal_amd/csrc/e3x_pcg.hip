@@ -20,6 +20,17 @@
 namespace e3x {
 namespace {
 
+// Cache hints of the streaming update (measured at 200^3): the FP32 form's read-once
+// inputs (Ah, M^-1) are loaded non-temporally (-8 % for the kernel); in FP64 the same hint
+// made it 14 % slower.  z is stored non-temporally in both forms: the next apply reads
+// it through its halo lines later, and the hint also made that apply ~1 % faster.
+template <typename T>
+__device__ __forceinline__ T load_once(const T *p)
+{
+    if constexpr (sizeof(T) == 4) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 template <typename T>
 struct PcgArgs {
     int64_t ns;
@@ -137,7 +148,7 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
         double mi[6];
         if constexpr (DO_Z) {
 #pragma unroll
-            for (int q = 0; q < 6; q++) mi[q] = (double)minv[6 * m + q];
+            for (int q = 0; q < 6; q++) mi[q] = (double)load_once(&minv[6 * m + q]);
         }
 #pragma unroll
         for (int r = 0; r < NR; r++) {
@@ -152,7 +163,7 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
             for (int c = 0; c < 3; c++) {
                 g[c] = (double)gv[o + c];
                 if constexpr (UPDATE_G) {
-                    const T gn = (T)(g[c] - lam[r] * (double)ah[o + c]);
+                    const T gn = (T)(g[c] - lam[r] * (double)load_once(&ah[o + c]));
                     gv[o + c] = gn;
                     g[c] = (double)gn;
                 }
@@ -163,7 +174,9 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
                 const double z1 = mi[3] * g[0] + mi[1] * g[1] + mi[5] * g[2];
                 const double z2 = mi[4] * g[0] + mi[5] * g[1] + mi[2] * g[2];
                 const T t0 = (T)z0, t1 = (T)z1, t2 = (T)z2;
-                zv[o] = t0; zv[o + 1] = t1; zv[o + 2] = t2;
+                __builtin_nontemporal_store(t0, &zv[o]);
+                __builtin_nontemporal_store(t1, &zv[o + 1]);
+                __builtin_nontemporal_store(t2, &zv[o + 2]);
                 part[r] += (double)t0 * g[0] + (double)t1 * g[1] + (double)t2 * g[2];
             }
         }
