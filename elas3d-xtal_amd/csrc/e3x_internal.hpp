@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace e3x {
@@ -17,6 +18,7 @@ constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched roun
 // 4x4x4 sub-cube.  (8 x 8 x 8 tiles, 512 threads, ran 8 % slower at 200^3.)
 constexpr int TX = 8, TY = 8, TZ = 4;
 constexpr int kSB = TX * TY * TZ;  // threads of a stencil block
+using PermT = std::conditional_t<(kSB <= 256), uint8_t, uint16_t>;  // a lane's node in its tile
 constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HN = HX * HY * HZ;  // node halo
 constexpr int VX = TX + 1, VY = TY + 1, VZ = TZ + 1, VN = VX * VY * VZ;  // voxel halo
 // Homogeneous 27-point stencil per phase, symmetric half: blocks of neighbours 14..26
@@ -58,7 +60,7 @@ struct Solver {
     // per stencil tile, the order in which the block's lanes take its nodes (node t of
     // the tile's wave-cube numbering): homogeneous nodes by phase, then mixed nodes by
     // their phases, so that waves hold few distinct phases; built on upload
-    uint8_t *perm = nullptr;                                   // [tiles][kSB]
+    PermT *perm = nullptr;                                     // [tiles][kSB]
 
     // fields, [nrhs][ns][3] (the three components of a node side by side)
     double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;

@@ -30,7 +30,7 @@ struct StencilArgs {
     int nx, ny, nz, ntx, nty, nrhs;
     int64_t ns;
     const uint16_t *pix;
-    const uint8_t *perm;   // lane order per tile (Solver::perm)
+    const PermT *perm;     // lane order per tile (Solver::perm)
     const T *dk, *sten;
     const T *x;            // [nrhs][ns][3]
     T *y;                  // [nrhs][ns][3] (kApply, kEnergy, kPlain)
@@ -61,6 +61,8 @@ struct HPos {
     int x, y, z;
 };
 constexpr int kShell = HN - kSB;
+constexpr int kShellPer = (kShell + kSB - 1) / kSB;  // shell positions per thread
+constexpr int kVoxPer = (VN + kSB - 1) / kSB;        // voxel-halo positions per thread
 __device__ __forceinline__ HPos inner_pos(int n) { return {1 + n % TX, 1 + (n / TX) % TY, 1 + n / (TX * TY)}; }
 __device__ __forceinline__ HPos shell_pos(int s)
 {
@@ -142,9 +144,9 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 
     // --- loads -------------------------------------------------------------------
     // voxel phases of the 9x9x5 voxel halo (positions tid and tid + kSB)
-    int pvl[2];
+    int pvl[kVoxPer];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < kVoxPer; k++) {
         const int q = min(tid + k * kSB, VN - 1);
         const HPos p = {q % VX, (q / VX) % VY, q / (VX * VY)};
         pvl[k] = pix[gpos(p)];
@@ -152,20 +154,32 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // this lane's node: t in the tile's wave-cube numbering (homogeneous nodes by phase,
     // then mixed ones by their phases; k_tile_order)
     const int t = A.perm[(int64_t)bid * kSB + tid];
-    // halo: the tile's own node tid and shell positions tid, tid + kSB
-    const HPos pin = inner_pos(tid), sh0 = shell_pos(tid), sh1 = shell_pos(tid + kSB);
-    const bool has1 = tid + kSB < kShell;
-    const int64_t gi = o + 3 * (int64_t)gpos(pin), g0 = o + 3 * (int64_t)gpos(sh0), g1 = o + 3 * (int64_t)gpos(has1 ? sh1 : sh0);
+    // halo: the tile's own node tid and shell positions tid + k kSB (a thread without
+    // one loads its own node again and does not store it)
+    const HPos pin = inner_pos(tid);
+    HPos shp[kShellPer];
+    bool shv[kShellPer];
+    int64_t gs[kShellPer];
+#pragma unroll
+    for (int k = 0; k < kShellPer; k++) {
+        shv[k] = tid + k * kSB < kShell;
+        shp[k] = shv[k] ? shell_pos(tid + k * kSB) : pin;
+        gs[k] = o + 3 * (int64_t)gpos(shp[k]);
+    }
+    const int64_t gi = o + 3 * (int64_t)gpos(pin);
     const bool ingrid = ox + pin.x - 1 < nx && oy + pin.y - 1 < ny && oz + pin.z - 1 < nz;
     const T *src = FUSED ? zv : x;
-    T ai[3], a0[3], a1[3];  // z (fused) or x
-    T bi[3], b0[3], b1[3];  // h_prev (fused)
+    T ai[3], as_[kShellPer][3];  // z (fused) or x
+    T bi[3], bs_[kShellPer][3];  // h_prev (fused)
     double ui[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        ai[c] = src[gi + c]; a0[c] = src[g0 + c]; a1[c] = src[g1 + c];
-        if constexpr (FUSED) {
-            bi[c] = hold[gi + c]; b0[c] = hold[g0 + c]; b1[c] = hold[g1 + c];
+        ai[c] = src[gi + c];
+        if constexpr (FUSED) bi[c] = hold[gi + c];
+#pragma unroll
+        for (int k = 0; k < kShellPer; k++) {
+            as_[k][c] = src[gs[k] + c];
+            if constexpr (FUSED) bs_[k][c] = hold[gs[k] + c];
         }
     }
     if constexpr (FUSED) {
@@ -177,10 +191,13 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 
     // --- stage ---------------------------------------------------------------------
 #pragma unroll
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < kVoxPer; k++)
         if (tid + k * kSB < VN) ps[tid + k * kSB] = pvl[k];
     if (stop != 0) return;  // uniform over the block
-    const int q = hpos_index(pin), q0 = hpos_index(sh0), q1 = hpos_index(sh1);
+    const int q = hpos_index(pin);
+    int qs[kShellPer];
+#pragma unroll
+    for (int k = 0; k < kShellPer; k++) qs[k] = hpos_index(shp[k]);
     if constexpr (FUSED) {
         // this iteration's direction, h = z + gamma h_prev or h = z on a reset
         // (f90:1961-1996); the tile's own nodes also store it and take the previous step
@@ -191,8 +208,9 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
         for (int c = 0; c < 3; c++) {
             const T hv = dir(ai[c], bi[c]);
             hs[c * HN + q] = hv;
-            hs[c * HN + q0] = dir(a0[c], b0[c]);
-            if (has1) hs[c * HN + q1] = dir(a1[c], b1[c]);
+#pragma unroll
+            for (int k = 0; k < kShellPer; k++)
+                if (shv[k]) hs[c * HN + qs[k]] = dir(as_[k][c], bs_[k][c]);
             if (ingrid) {
                 hnew[gi + c] = hv;
                 if (dou) u[gi + c] = ui[c] - lam * (double)bi[c];
@@ -202,8 +220,9 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             hs[c * HN + q] = ai[c];
-            hs[c * HN + q0] = a0[c];
-            if (has1) hs[c * HN + q1] = a1[c];
+#pragma unroll
+            for (int k = 0; k < kShellPer; k++)
+                if (shv[k]) hs[c * HN + qs[k]] = as_[k][c];
         }
     }
     __syncthreads();
@@ -362,9 +381,9 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 // Lane order of each stencil tile (Solver::perm), from the phases alone: one block per
 // tile, thread t = node t of the tile's wave-cube numbering.  Key: homogeneous nodes
 // first, by phase; then mixed nodes by (smallest, largest, octant-0 phase); nodes
-// outside the grid last; ties by t.  A bitonic sort of the 256 keys in LDS.
+// outside the grid last; ties by t.  A bitonic sort of the kSB keys in LDS.
 __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int ntx, int nty, const uint16_t *pix,
-                                                  uint8_t *perm)
+                                                  PermT *perm)
 {
     __shared__ int ps[VN];
     __shared__ unsigned long long key[kSB];
@@ -403,7 +422,7 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
             }
             __syncthreads();
         }
-    perm[(int64_t)bid * kSB + t] = (uint8_t)(key[t] & 0xFFull);
+    perm[(int64_t)bid * kSB + t] = (PermT)(key[t] & 0xFFFull);
 }
 
 template <int NR>
