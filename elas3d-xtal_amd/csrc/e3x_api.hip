@@ -362,6 +362,7 @@ int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const doubl
     E3X_HIP(launch_pcg_init(s, mixed, kkk, ldemb, tol, gg0));
     Solver::PcgRun &R = s.run;
     R = Solver::PcgRun{};
+    s.pcg_known_stopped = 0;
     R.on = true;
     R.mixed = mixed;
     R.replace_every = mixed ? replace_every : 0;
@@ -410,7 +411,11 @@ int e3x_dembx_advance(e3x_solver *h, int block, int *done)
                 E3X_HIP(q);
             }
             bool all = true;
-            for (int r = 0; r < s.nrhs; r++) all &= flags[R.checked & 1][r] != 0;
+            for (int r = 0; r < s.nrhs; r++) {
+                const bool stopped = flags[R.checked & 1][r] != 0;
+                all &= stopped;
+                if (stopped) s.pcg_known_stopped |= 1u << r;
+            }
             R.checked++;
             if (all) R.stop_seen = true;
             continue;

@@ -104,22 +104,31 @@ __device__ __forceinline__ bool last_wave(unsigned *arrived)
 constexpr int kFinBlock = 1024;
 
 // In a finalize kernel of kFinBlock threads: fixed-order totals of the partials, valid
-// in thread 0.  Each thread keeps 8 independent loads in flight.
+// in thread 0.  Each thread keeps U loads in flight (U = 32 for one or two sums, so that
+// the 31,250 partials of a 200^3 apply come in one round of memory latency; 8 for the
+// wide finalizers), adds them in a fixed tree, then the block sum.
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double *partials, int nparts, double (&tot)[NV])
 {
+    constexpr int U = NV <= 2 ? 32 : 8;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         const double *p = partials + (int64_t)q * nparts;
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int i = threadIdx.x; i < nparts; i += 8 * kFinBlock) {
+        double acc[U];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < U; u++) acc[u] = 0.0;
+        for (int i = threadIdx.x; i < nparts; i += U * kFinBlock) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
                 const int idx = i + u * kFinBlock;
                 if (idx < nparts) acc[u] += p[idx];
             }
         }
-        tot[q] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+#pragma unroll
+        for (int w = U / 2; w >= 1; w /= 2)
+#pragma unroll
+            for (int u = 0; u < w; u++) acc[u] += acc[u + w];
+        tot[q] = acc[0];
     }
     block_sum<NV, kFinBlock>(tot);
 }

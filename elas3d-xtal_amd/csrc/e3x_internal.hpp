@@ -93,6 +93,9 @@ struct Solver {
         bool on = false, mixed = false, stop_seen = false, finishing = false;
         int replace_every = 0, ldemb = 0, chunk = 0, it = 1, k = 0, checked = 0;
     } run;
+    // right-hand sides the host has seen stop in the PCG in progress (bit r): the apply
+    // launches no blocks for them
+    uint32_t pcg_known_stopped = 0;
 
     bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
     double last_precond_ms = 0, last_pcg_ms = 0;

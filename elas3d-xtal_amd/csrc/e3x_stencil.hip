@@ -44,6 +44,7 @@ struct StencilArgs {
     T *hnew;
     double *u;
     int fuse_u;
+    int rmap[kMaxRhs];     // grid row j serves right-hand side rmap[j]
 };
 
 __device__ __forceinline__ int pmod(int v, int n) { v %= n; return v < 0 ? v + n : v; }
@@ -117,7 +118,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // remainder mod 8 is the XCD, and (tile, right-hand side) come from the remapped id
     const int ntile = gridDim.x, nrhs = A.nrhs;
     const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
-    const int r = lin / ntile, bid = lin % ntile;
+    const int r = A.rmap[lin / ntile], bid = lin % ntile;
     const int tx = bid % A.ntx, ty = (bid / A.ntx) % A.nty, tz = bid / (A.ntx * A.nty);
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
@@ -446,6 +447,7 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
     A.ntx = (s.nx + TX - 1) / TX; A.nty = (s.ny + TY - 1) / TY;
     A.ns = s.ns; A.pix = s.pix; A.perm = s.perm; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
     A.partials = s.partials; A.st = s.st;
+    for (int r = 0; r < kMaxRhs; r++) A.rmap[r] = r;
     return A;
 }
 
@@ -454,9 +456,9 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
 // 200^3: the per-rhs FMA, LDS and staging work dominates, and the second halo costs
 // occupancy.)
 template <typename T, int MODE>
-void dispatch_stencil(const Solver &s, const StencilArgs<T> &A)
+void dispatch_stencil(const Solver &s, const StencilArgs<T> &A, int rows = -1)
 {
-    const dim3 grid((unsigned)stencil_blocks(s), (unsigned)s.nrhs), block(kSB);
+    const dim3 grid((unsigned)stencil_blocks(s), (unsigned)(rows < 0 ? s.nrhs : rows)), block(kSB);
     if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_stencil<T, MODE, 8>), grid, block, 0, s.stream, A);
     else hipLaunchKernelGGL((k_stencil<T, MODE, 6>), grid, block, 0, s.stream, A);
 }
@@ -497,28 +499,36 @@ hipError_t launch_apply_plain(Solver &s, const double *x, double *y)
 }
 
 // Iteration 1 applies K to h as it stands; iteration it >= 2 forms its direction from
-// z and the previous one (h_alt / h by parity) while staging.
+// z and the previous one (h_alt / h by parity) while staging.  The grid has a row for
+// each right-hand side not yet known (on the host) to have stopped; the blocks of one
+// that stopped since still find that out from the device state.
 template <typename T>
-void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T *ah, int it, bool fuse_u)
+void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T *ah, int it, bool fuse_u,
+                const int (&rmap)[kMaxRhs], int rows)
 {
+    StencilArgs<T> A = stencil_args<T>(s, dk, sten, it <= 1 ? hbuf[1] : nullptr, ah);
+    for (int j = 0; j < kMaxRhs; j++) A.rmap[j] = rmap[j];
     if (it <= 1) {
-        dispatch_stencil<T, kApply>(s, stencil_args<T>(s, dk, sten, hbuf[1], ah));
+        dispatch_stencil<T, kApply>(s, A, rows);
     } else {
-        StencilArgs<T> A = stencil_args<T>(s, dk, sten, nullptr, ah);
         A.z = z; A.hold = hbuf[(it - 1) & 1]; A.hnew = hbuf[it & 1];
         A.u = s.u; A.fuse_u = fuse_u ? 1 : 0;
-        dispatch_stencil<T, kApplyFused>(s, A);
+        dispatch_stencil<T, kApplyFused>(s, A, rows);
     }
 }
 
 hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u)
 {
+    int rmap[kMaxRhs] = {}, rows = 0;
+    for (int r = 0; r < s.nrhs; r++)
+        if (!((s.pcg_known_stopped >> r) & 1u)) rmap[rows++] = r;
+    if (rows == 0) return hipSuccess;  // every right-hand side has stopped: nothing to apply
     if (mixed) {
         float *hb[2] = {s.h32_alt, s.h32};
-        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, it, fuse_u);
+        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, it, fuse_u, rmap, rows);
     } else {
         double *hb[2] = {s.h_alt, s.h};
-        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, it, fuse_u);
+        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, it, fuse_u, rmap, rows);
     }
     return launch_hah_finish(s, (int)stencil_parts(s), it);
 }
@@ -527,12 +537,14 @@ hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u)
 // left to the caller: with lambda = gamma = 0 it leaves u unchanged and sets h = z).
 hipError_t launch_apply_only(Solver &s, bool mixed)
 {
+    int all[kMaxRhs];
+    for (int r = 0; r < kMaxRhs; r++) all[r] = r;
     if (mixed) {
         float *hb[2] = {s.h32_alt, s.h32};
-        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, 2, true);
+        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, 2, true, all, s.nrhs);
     } else {
         double *hb[2] = {s.h_alt, s.h};
-        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, 2, true);
+        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, 2, true, all, s.nrhs);
     }
     return hipGetLastError();
 }
