@@ -44,3 +44,12 @@ def test_version_and_error_string_without_gpu():
     lib = L.load()
     assert lib.e3x_version() >= 10000
     assert isinstance(lib.e3x_last_error(), bytes)
+
+
+def test_one_node_thick_grid_is_rejected_without_gpu():
+    """e3x_create checks the extents before it touches a device (include/e3x.h)."""
+    from elas3dxtal.solver import Elas3D
+    for shape in [(1, 4, 4), (4, 1, 4), (4, 4, 1)]:
+        with pytest.raises(L.E3xError) as e:
+            Elas3D(shape, 2)
+        assert e.value.code == L.E3X_EINVAL

@@ -141,6 +141,34 @@ def test_dembx_matches_oracle(shape, ngrain):
         assert _rel(h, r_o["h"]) < 1e-5
 
 
+@pytest.mark.parametrize("shape", [(2, 2, 2), (2, 3, 2), (3, 2, 5), (2, 9, 2), (5, 2, 3)])
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_thin_grids_match_oracle(shape, precision):
+    """Grids thinner than a stencil tile in every direction, down to two nodes: a node's
+    -1 and +1 neighbours in a 2-wide direction are the same node (f90:350-372), and one
+    tile covers the grid several times over through the periodic wrap."""
+    pix, rod = voronoi_case(shape, 2, seed=7)
+    e6 = [oc.AML_F32, 0, 0, 0, 0.3e-3, 0]
+    r_o, gg0_o = _oracle_pcg(shape, pix, rod, e6)
+    assert r_o["status"] == oc.OC_CONVERGED
+    with Elas3D(shape, 3) as s:
+        s.ppixel(pix, rod)
+        s.femat(C11, C12, C44, e6)
+        s.affine_guess()
+        u0 = s.get(L.FIELD_U)
+        _, gg0 = s.energy()
+        d = s.dembx(gg0, precision=precision)
+        u = s.get(L.FIELD_U)
+    assert abs(gg0[0] - gg0_o) < 1e-11 * gg0_o
+    assert d["exit"][0] == L.PCG_CONVERGED
+    if precision == L.PREC_FP64:
+        assert d["lstep"][0] == r_o["lstep"]
+        assert _rel(u, r_o["u"]) < 1e-10
+    else:
+        assert abs(int(d["lstep"][0]) - r_o["lstep"]) <= 2
+        assert _rel(u - u0, r_o["u"] - u0) < 1e-5
+
+
 def test_full_solve_matches_oracle_config1_shape():
     """Plain FP64 PCG on a 50^3 periodic cube: 20 Voronoi grains, 2% single-voxel pores,
     C11=246.5, C12=147.3, C44=124.7 GPa, exx = float32(1e-3)."""
