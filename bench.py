@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 APPLY_BYTES_PER_NODE = {L.PREC_FP64: 6 * 3 * 8 + 2, L.PREC_MIXED: 4 * 3 * 4 + 2 * 3 * 8 + 2}
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "stencil_apply_cfg2_200_pmc.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "stencil_apply_cube200_pmc.txt")
 
 
 def measured_traffic(key):
@@ -207,8 +207,8 @@ def profile_run(n, zener):
     counter readings calibrate the bytes."""
     import torch
 
-    w = configs.build("cfg2", n=n)
-    c11, c12, c44 = configs.materials("cfg2", zener=zener)
+    w = configs.build("cube200", n=n)
+    c11, c12, c44 = configs.materials("cube200", zener=zener)
     ns = int(np.prod(w["shape"]))
     with Elas3D(w["shape"], w["nphase"]) as s:
         s.ppixel(w["pix"], w["rod"])
@@ -255,8 +255,8 @@ def main():
     if args.profile:
         profile_run(args.n, args.zener)
         return
-    w = configs.build("cfg2", n=args.n)
-    c11, c12, c44 = configs.materials("cfg2", zener=args.zener)
+    w = configs.build("cube200", n=args.n)
+    c11, c12, c44 = configs.materials("cube200", zener=args.zener)
     e6 = configs.exx_strain()
     ns = int(np.prod(w["shape"]))
 

@@ -1,4 +1,4 @@
-"""The five benchmark workloads of the elas3dxtal_pcg solve, as generator parameters and
+"""The benchmark workloads of the elas3dxtal_pcg solve, as generator parameters and
 material constants.  Moduli in Pa; strains are tensor components {exx,eyy,ezz,exz,eyz,exy}."""
 import numpy as np
 
@@ -15,20 +15,20 @@ def c11_for_zener(a, c12=C12_GPA, c44=C44_GPA):
 
 CONFIGS = {
     # Plain FP64 PCG, 50^3, 20 grains, 2% random single-voxel pores, exx only.
-    "cfg1": dict(n=50, ngrain=20, c=(246.5, 147.3, 124.7), pores=dict(pore_mode=io.PORES_VOXELS, pore_fraction=0.02),
+    "cube50": dict(n=50, ngrain=20, c=(246.5, 147.3, 124.7), pores=dict(pore_mode=io.PORES_VOXELS, pore_fraction=0.02),
                  seed=1001),
     # Plain FP64 PCG, 200^3, 500 grains, Zener A in {0.5, 1, 2, 5} (A=2 is the headline),
     # 1% lognormal spheres (median 3 voxels, sigma 0.4), exx.
-    "cfg2": dict(n=200, ngrain=500, zener=2.0,
+    "cube200": dict(n=200, ngrain=500, zener=2.0,
                  pores=dict(pore_mode=io.PORES_SPHERES, pore_fraction=0.01, pore_median=3.0, pore_sigma=0.4),
                  seed=2001),
     # Mixed precision, 300^3, 1500 grains, A=2, 3% spheres, reliable update every 50.
-    "cfg3": dict(n=300, ngrain=1500, zener=2.0,
+    "cube300": dict(n=300, ngrain=1500, zener=2.0,
                  pores=dict(pore_mode=io.PORES_SPHERES, pore_fraction=0.03, pore_median=3.0, pore_sigma=0.4),
                  seed=3001),
     # Six unit load cases, 400^3, 4000 grains, A=3, 2% spheres + lack-of-fusion ellipsoids
     # (semi-axes 20 x 5 x 3 voxels, random yaw about z).
-    "cfg4": dict(n=400, ngrain=4000, zener=3.0,
+    "cube400": dict(n=400, ngrain=4000, zener=3.0,
                  pores=dict(pore_mode=io.PORES_SPHERES, pore_fraction=0.02, pore_median=3.0, pore_sigma=0.4,
                             lof_fraction=0.005, lof_axes=(20.0, 5.0, 3.0)),
                  seed=4001),

@@ -169,7 +169,7 @@ def test_thin_grids_match_oracle(shape, precision):
         assert _rel(u - u0, r_o["u"] - u0) < 1e-5
 
 
-def test_full_solve_matches_oracle_config1_shape():
+def test_full_solve_matches_oracle_cube50():
     """Plain FP64 PCG on a 50^3 periodic cube: 20 Voronoi grains, 2% single-voxel pores,
     C11=246.5, C12=147.3, C44=124.7 GPa, exx = float32(1e-3)."""
     shape = (50, 50, 50)

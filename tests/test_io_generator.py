@@ -162,9 +162,9 @@ def test_hdf5_fullfield_round_trip(tmp_path):
 
 
 def test_config_materials():
-    c11, c12, c44 = configs.materials("cfg2")
+    c11, c12, c44 = configs.materials("cube200")
     assert abs(2 * c44 / (c11 - c12) - 2.0) < 1e-12
-    assert configs.materials("cfg1") == (246.5e9, 147.3e9, 124.7e9)
+    assert configs.materials("cube50") == (246.5e9, 147.3e9, 124.7e9)
     for a, want in [(0.5, 646.1), (1.0, 396.7), (5.0, 197.18)]:
         assert abs(configs.c11_for_zener(a) - want) < 1e-9
     e = configs.six_unit_strains()
