@@ -53,3 +53,11 @@ def test_one_node_thick_grid_is_rejected_without_gpu():
         with pytest.raises(L.E3xError) as e:
             Elas3D(shape, 2)
         assert e.value.code == L.E3X_EINVAL
+
+
+def test_grid_beyond_int32_nodes_is_rejected_without_gpu():
+    """Node indices inside the kernels are 32-bit: e3x_create refuses 2^31 nodes or more."""
+    from elas3dxtal.solver import Elas3D
+    with pytest.raises(L.E3xError) as e:
+        Elas3D((1291, 1291, 1291), 2)
+    assert e.value.code == L.E3X_EINVAL
