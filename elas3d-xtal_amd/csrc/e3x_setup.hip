@@ -3,6 +3,8 @@
 // block-Jacobi preconditioner.
 #include "e3x_device.hpp"
 
+#include <algorithm>
+
 namespace e3x {
 namespace {
 
@@ -213,10 +215,40 @@ __device__ __forceinline__ double jump_delta(const JumpArgs &A, int r, int fx, i
     return s;
 }
 
+// Only nodes on the grid's faces touch a voxel whose upper face is a periodic boundary
+// (i, j or k at 0 or at the last index); the rest of b is zero.  Face node number e: the
+// planes k = 0 and k = nz-1, then j = 0 and j = ny-1 with 0 < k < nz-1, then i = 0 and
+// i = nx-1 with 0 < j < ny-1, 0 < k < nz-1 (each face node once, for extents >= 2).
+__host__ __device__ inline int64_t face_count(int nx, int ny, int nz)
+{
+    return 2 * (int64_t)nx * ny + 2 * (int64_t)nx * (nz - 2) + 2 * (int64_t)(ny - 2) * (nz - 2);
+}
+
+__device__ __forceinline__ int64_t face_node(int64_t e, int nx, int ny, int nz)
+{
+    const int64_t nxy = (int64_t)nx * ny;
+    if (e < 2 * nxy) return e % nxy + nxy * (e < nxy ? 0 : nz - 1);
+    e -= 2 * nxy;
+    const int64_t sj = (int64_t)nx * (nz - 2);
+    if (e < 2 * sj) {
+        const int j = e < sj ? 0 : ny - 1;
+        const int64_t r = e % sj;
+        return r % nx + nx * (j + (int64_t)ny * (1 + r / nx));
+    }
+    e -= 2 * sj;
+    const int64_t si = (int64_t)(ny - 2) * (nz - 2);
+    const int i = e < si ? 0 : nx - 1;
+    const int64_t r = e % si;
+    return i + nx * (1 + r % (ny - 2) + (int64_t)ny * (1 + r / (ny - 2)));
+}
+
+// b at the face nodes (the caller zeroes b first) and the constants of their voxels.
 __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *partials)
 {
     double cpart[kMaxRhs] = {};
-    for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < A.ns; m += (int64_t)gridDim.x * kBlock) {
+    const int64_t nface = face_count(A.nx, A.ny, A.nz);
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nface; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t m = face_node(e, A.nx, A.ny, A.nz);
         const int i = (int)(m % A.nx), j = (int)((m / A.nx) % A.ny), k = (int)(m / ((int64_t)A.nx * A.ny));
         for (int r = 0; r < A.nrhs; r++) {
             double acc[3] = {0.0, 0.0, 0.0};
@@ -346,7 +378,10 @@ hipError_t launch_femat(Solver &s, double c11, double c12, double c44, const dou
 
 hipError_t launch_rhs(Solver &s)
 {
-    const int g = (int)vector_grid(s);
+    const int64_t nface = face_count(s.nx, s.ny, s.nz);
+    const int g = (int)std::min<int64_t>(vector_grid(s), (nface + kBlock - 1) / kBlock);
+    hipError_t e = hipMemsetAsync(s.b, 0, sizeof(double) * 3 * (size_t)s.nrhs * (size_t)s.ns, s.stream);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_rhs, dim3(g), dim3(kBlock), 0, s.stream, jump_args(s), s.b, s.partials);
     hipLaunchKernelGGL(k_rhs_finish, dim3(1), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.nrhs,
                        s.cconst);
