@@ -121,6 +121,29 @@ __global__ __launch_bounds__(kFinBlock) void k_hah_finish(const double *partials
     }
 }
 
+// First stage of the h.Ah sum for large grids: block (g, r) adds its slice of right-hand
+// side r's apply partials in a fixed order and writes out[r * gridDim.x + g] (a single
+// finalize block alone keeps too few loads in flight for 31,250 partials).
+__global__ __launch_bounds__(kBlock) void k_hah_stage(const double *partials, int nparts, const PcgState *st,
+                                                    double *out)
+{
+    const int r = blockIdx.y, G = gridDim.x, g = blockIdx.x;
+    if (st[r].stop_iter != 0) return;  // uniform over the block
+    const int chunk = (nparts + G - 1) / G, lo = g * chunk, hi = min(nparts, lo + chunk);
+    const double *p = partials + (int64_t)r * nparts;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = lo + (int)threadIdx.x; i < hi; i += 4 * kBlock) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int idx = i + u * kBlock;
+            if (idx < hi) acc[u] += p[idx];
+        }
+    }
+    double v[1] = {(acc[0] + acc[1]) + (acc[2] + acc[3])};
+    block_sum<1, kBlock>(v);
+    if (threadIdx.x == 0) out[(int64_t)r * G + g] = v[0];
+}
+
 // g -= lambda Ah (unless the true gradient replaces g this iteration); z = M^-1 g;
 // partials of z.g (f90:1919-1946).  DO_U: u -= lambda h here (iterations whose true
 // gradient needs the current u); otherwise the next iteration's apply moves u.
@@ -379,8 +402,17 @@ void finalize_t(Solver &s, int replace_every)
 
 hipError_t launch_hah_finish(Solver &s, int nparts, int it)
 {
-    hipLaunchKernelGGL(k_hah_finish, dim3((unsigned)s.nrhs), dim3(kFinBlock), 0, s.stream, (const double *)s.partials,
-                       nparts, s.st, it);
+    const double *p = s.partials;
+    constexpr int kStage = 64;
+    if (nparts > 4096 && (int64_t)s.nrhs * (nparts + kStage) <= s.partials_cap) {
+        // two stages: kStage partial sums per right-hand side behind the apply's partials
+        double *stage = s.partials + (int64_t)s.nrhs * nparts;
+        hipLaunchKernelGGL(k_hah_stage, dim3(kStage, (unsigned)s.nrhs), dim3(kBlock), 0, s.stream, p, nparts,
+                           (const PcgState *)s.st, stage);
+        p = stage;
+        nparts = kStage;
+    }
+    hipLaunchKernelGGL(k_hah_finish, dim3((unsigned)s.nrhs), dim3(kFinBlock), 0, s.stream, p, nparts, s.st, it);
     return hipGetLastError();
 }
 
