@@ -293,7 +293,8 @@ def main():
         "config": {"workload": f"{args.n}^3 periodic cube, 500 Voronoi grains, Zener A={args.zener:g}, "
                                "1% lognormal spherical pores, exx=1e-3, reference PCG controls",
                    "grid": list(w["shape"]), "grains": 500, "zener": args.zener,
-                   "pore_fraction": w["pore_fraction"], "dof": 3 * ns},
+                   "pore_fraction": w["pore_fraction"], "dof": 3 * ns,
+                   "stop_rule": "reference: sqrt(z.g/gg0) < float32(1e-8)"},
         "lstep": fp64["lsteps"][0],
         "exit": fp64["exit"],
         "solves_per_s": float(1.0 / walls.mean()),

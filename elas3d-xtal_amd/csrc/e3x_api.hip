@@ -327,6 +327,15 @@ int e3x_energy(e3x_solver *h, double *utot, double *gg)
     return E3X_OK;
 }
 
+int e3x_set_stop_rule(e3x_solver *h, int rule)
+{
+    if (int rc = check_idle(h)) return rc;
+    if (rule != E3X_STOP_REFERENCE && rule != E3X_STOP_NIST_ABS)
+        return fail(E3X_EINVAL, "rule must be E3X_STOP_REFERENCE or E3X_STOP_NIST_ABS");
+    S(h).stop_rule = rule;
+    return E3X_OK;
+}
+
 int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const double *gg0, float tol, int ldemb,
                     int kkk)
 {

@@ -35,7 +35,10 @@ struct PcgState {
     int reset_count;
     int run_dir;      // the stopping iteration still updates h (f90:2007 follows 1996)
     int ldemb;
-    int pad[2];
+    int stop_rule;    // e3x_stop_rule
+    int pad;
+    double gsq_tol;   // E3X_STOP_NIST_ABS: stop when the sum of g^2 falls below this
+    double gsq;       // the last iteration's sum of g^2 (E3X_STOP_NIST_ABS)
 };
 
 // kApplyFused: the PCG apply of iterations >= 2, which first forms this iteration's
@@ -96,6 +99,7 @@ struct Solver {
     // right-hand sides the host has seen stop in the PCG in progress (bit r): the apply
     // launches no blocks for them
     uint32_t pcg_known_stopped = 0;
+    int stop_rule = 0;  // e3x_stop_rule of the next PCG
 
     bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
     double last_precond_ms = 0, last_pcg_ms = 0;

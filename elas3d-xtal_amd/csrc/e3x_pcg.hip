@@ -146,8 +146,9 @@ __global__ __launch_bounds__(kBlock) void k_hah_stage(const double *partials, in
 
 // g -= lambda Ah (unless the true gradient replaces g this iteration); z = M^-1 g;
 // partials of z.g (f90:1919-1946).  DO_U: u -= lambda h here (iterations whose true
-// gradient needs the current u); otherwise the next iteration's apply moves u.
-template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z>
+// gradient needs the current u); otherwise the next iteration's apply moves u.  GSQ:
+// also the partials of g.g, in slots [NR + r] (E3X_STOP_NIST_ABS).
+template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z, bool GSQ = false>
 __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__restrict__ u, T *__restrict__ gv,
                                                 const T *__restrict__ hv, T *__restrict__ zv,
                                                 const T *__restrict__ ah, const T *__restrict__ minv)
@@ -163,9 +164,9 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
     }
     if (!any) return;
     const int64_t ns = A.ns;
-    double part[NR];
+    double part[GSQ ? 2 * NR : NR];
 #pragma unroll
-    for (int r = 0; r < NR; r++) part[r] = 0.0;
+    for (int q = 0; q < (GSQ ? 2 * NR : NR); q++) part[q] = 0.0;
     for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
         // the node's M^-1 block, once for all right-hand sides
         double mi[6];
@@ -201,29 +202,32 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
                 __builtin_nontemporal_store(t1, &zv[o + 1]);
                 __builtin_nontemporal_store(t2, &zv[o + 2]);
                 part[r] += (double)t0 * g[0] + (double)t1 * g[1] + (double)t2 * g[2];
+                if constexpr (GSQ) part[NR + r] += g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
             }
         }
     }
-    if constexpr (DO_Z) write_partials<NR>(part, A.partials);
+    if constexpr (DO_Z) write_partials<GSQ ? 2 * NR : NR>(part, A.partials);
 }
 
-template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z>
+template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z, bool GSQ = false>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
 {
-    pcg_update_body<T, NR, UPDATE_G, DO_U, DO_Z>(A, A.u, A.g, A.h, A.z, A.ah, A.minv);
+    pcg_update_body<T, NR, UPDATE_G, DO_U, DO_Z, GSQ>(A, A.u, A.g, A.h, A.z, A.ah, A.minv);
 }
 
 // gg, rel_res, history, divergence reset or gamma, stop test (f90:1938-2007); block r
 // serves right-hand side r
 __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *partials, int nparts, PcgState *st,
-                                                          double *hist, int it)
+                                                          double *hist, int it, int nrhs)
 {
     const int r = blockIdx.x;
     PcgState &S = st[r];
     if (S.stop_iter != 0) return;  // uniform over the block
-    double tot[1];
+    double tot[1], gsq[1] = {0.0};
     sum_partials<1>(partials + (int64_t)r * nparts, nparts, tot);
+    if (S.stop_rule == 1) sum_partials<1>(partials + (int64_t)(nrhs + r) * nparts, nparts, gsq);  // uniform
     if (threadIdx.x != 0) return;
+    if (S.stop_rule == 1) S.gsq = gsq[0];
     const double gglast = S.gg;
     S.gg = tot[0];
     const double rel = sqrt(S.gg / S.gg0);
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *parti
         if (rel < S.rel_res_min) S.rel_res_min = rel;
     }
     S.rel_res_prev = rel;
-    if (rel < S.tol) {
+    if (S.stop_rule == 1 ? gsq[0] < S.gsq_tol : rel < S.tol) {
         S.stop_iter = it;
         S.exit_flag = 1;  // E3X_PCG_CONVERGED
         S.run_dir = 1;
@@ -334,8 +338,10 @@ void iter_nr(Solver &s, int it, bool replace, bool prev_replace)
     (void)launch_apply_hah(s, sizeof(T) == 4, it, !prev_replace);
     PcgArgs<T> A = pcg_args<T>(s, it);
     // 2. updates, z and z.g (f90:1914-2007)
+    const bool gsq = s.stop_rule == 1;
     if (!replace) {
-        hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true>), vg, blk, 0, s.stream, A);
+        if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true, true>), vg, blk, 0, s.stream, A);
+        else hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true>), vg, blk, 0, s.stream, A);
     } else {
         // reliable update: u first (with this iteration's direction), then the true
         // gradient g = K u + b in FP64, then z
@@ -343,9 +349,11 @@ void iter_nr(Solver &s, int it, bool replace, bool prev_replace)
         else A.h = (it & 1) ? s.h : s.h_alt;
         hipLaunchKernelGGL((k_pcg_update<T, NR, false, true, false>), vg, blk, 0, s.stream, A);
         (void)launch_resid(s);
-        hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true>), vg, blk, 0, s.stream, A);
+        if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true, true>), vg, blk, 0, s.stream, A);
+        else hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true>), vg, blk, 0, s.stream, A);
     }
-    hipLaunchKernelGGL(k_update_finish, dim3(NR), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.st, s.hist, it);
+    hipLaunchKernelGGL(k_update_finish, dim3(NR), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.st,
+                       s.hist, it, NR);
 }
 
 template <typename T, int NR>
@@ -425,6 +433,8 @@ hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
         hs[r].gg0 = gg0[r];
         hs[r].tol = (double)tol;
         hs[r].ldemb = ldemb;
+        hs[r].stop_rule = s.stop_rule;
+        hs[r].gsq_tol = 1e-8 * (double)s.ns;  // NIST ELAS3D: gtest = 1e-8 * (number of voxels)
     }
     hipError_t e = hipMemcpyAsync(s.st, hs, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return e;

@@ -43,6 +43,7 @@ class Elas3D:
         self._lib = lib
         self.strain6 = np.zeros((self.nrhs, 6))
         self.C = np.zeros(self.nrhs)
+        self.stop_rule = L.STOP_REFERENCE
 
     # -- lifetime -------------------------------------------------------------------
     def close(self):
@@ -107,6 +108,14 @@ class Elas3D:
         ut, gg = np.zeros(self.nrhs), np.zeros(self.nrhs)
         L.check(self._lib.e3x_energy(self._h, L.dptr(ut), L.dptr(gg)))
         return ut, gg
+
+    def set_stop_rule(self, rule):
+        """Stop test of the following PCG runs (e3x_set_stop_rule): L.STOP_REFERENCE, the
+        reference's relative residual rel_res < tol (f90:1996-2004, the default), or
+        L.STOP_NIST_ABS, NIST ELAS3D's absolute test sum(g^2) < 1e-8 * ns on the
+        unpreconditioned gradient (tol is then unused)."""
+        L.check(self._lib.e3x_set_stop_rule(self._h, rule))
+        self.stop_rule = rule
 
     def dembx(self, gg0, tol=TOL_DEFAULT, ldemb=LDEMB_DEFAULT, kkk=1, precision=L.PREC_FP64,
               replace_every=50, want_history=True):
@@ -203,15 +212,18 @@ class SolveResult:
 
 def solve(pix, orientation, shape, nphase, c11, c12, c44, strain6, phasemod=(0.0, 0.0),
           flag_m=1, tol=TOL_DEFAULT, ldemb=LDEMB_DEFAULT, precision=L.PREC_FP64,
-          replace_every=50, device=0, keep=False):
+          replace_every=50, device=0, keep=False, stop_rule=L.STOP_REFERENCE):
     """The main program's sequence (f90:326-540) for one microstructure.
 
     strain6 may be (6,) or (nrhs, 6); with nrhs > 1 the load cases are solved in one
-    batched PCG.  Returns SolveResult (and the live Elas3D if keep=True).
+    batched PCG.  stop_rule selects the PCG's stop test (Elas3D.set_stop_rule).
+    Returns SolveResult (and the live Elas3D if keep=True).
     """
     e = np.atleast_2d(np.asarray(strain6, float))
     s = Elas3D(shape, nphase, nrhs=e.shape[0], device=device)
     try:
+        if stop_rule != L.STOP_REFERENCE:
+            s.set_stop_rule(stop_rule)
         s.ppixel(pix, orientation)
         prob = s.assig()
         cval = s.femat(c11, c12, c44, e, phasemod=phasemod, flag_m=flag_m)

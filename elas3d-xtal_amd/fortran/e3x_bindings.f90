@@ -13,6 +13,7 @@
       integer(c_int), parameter :: E3X_PCG_CONVERGED = 1, E3X_PCG_HAH_BREAK = 2, E3X_PCG_MAXITER = 3
       integer(c_int), parameter :: E3X_FIELD_U = 0, E3X_FIELD_GB = 1, E3X_FIELD_B = 2, E3X_FIELD_H = 3
       integer(c_int), parameter :: E3X_PREC_FP64 = 0, E3X_PREC_MIXED = 1
+      integer(c_int), parameter :: E3X_STOP_REFERENCE = 0, E3X_STOP_NIST_ABS = 1
 
       interface
         function e3x_last_error() bind(C, name="e3x_last_error") result(p)
@@ -76,6 +77,14 @@
           import :: c_int, c_ptr, c_double
           type(c_ptr), value :: h
           real(c_double) :: utot(*), gg(*)
+        end function
+
+        ! stop test of the following dembx calls: the reference's rel_res < tol
+        ! (f90:1996-2004, default) or NIST ELAS3D's sum(gb**2) < 1e-8*ns
+        integer(c_int) function e3x_set_stop_rule(h, rule) bind(C, name="e3x_set_stop_rule")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+          integer(c_int), value :: rule
         end function
 
         ! dembx_OpenMP (f90:1356-2043), with a precision mode

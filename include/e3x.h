@@ -48,6 +48,15 @@ enum e3x_pcg_exit {
     E3X_PCG_MAXITER = 3    /* ldemb iterations without convergence          */
 };
 
+/* PCG stop rule (e3x_set_stop_rule). */
+enum e3x_stop_rule {
+    E3X_STOP_REFERENCE = 0, /* sqrt(z.g / gg0) < tol, z = M^-1 g (f90:1949, 2007): default */
+    E3X_STOP_NIST_ABS = 1   /* sum of g^2 < 1e-8 * ns on the unpreconditioned gradient:
+                               NIST ELAS3D's absolute rule, the wording of the reference's
+                               README; meaningful for moduli of order one (GPa units),
+                               not for moduli in Pa */
+};
+
 /* Fields reachable with e3x_get_field / e3x_set_field. */
 enum e3x_field {
     E3X_FIELD_U = 0,  /* displacement u(ns,3)          (e3d%u)  */
@@ -148,6 +157,14 @@ int e3x_dembx(e3x_solver *s, const double *gg0, float tol, int ldemb, int kkk, i
 int e3x_dembx_ex(e3x_solver *s, int precision, int replace_every, const double *gg0, float tol,
                  int ldemb, int kkk, int *lstep, double *gg, double *history, int *exit_flag,
                  int *reset_count);
+
+/*
+ * Stop rule of the handle's following e3x_dembx / e3x_dembx_ex / e3x_dembx_begin calls
+ * (e3x_stop_rule; E3X_STOP_REFERENCE until set).  With E3X_STOP_NIST_ABS the relative
+ * residual is still recorded in the history and `tol` is not used.  No counterpart in
+ * the reference, whose code has only the relative rule (f90:2007).
+ */
+int e3x_set_stop_rule(e3x_solver *s, int rule);
 
 /*
  * e3x_dembx_ex in three steps, so that one host thread can keep several handles'

@@ -61,3 +61,9 @@ def test_grid_beyond_int32_nodes_is_rejected_without_gpu():
     with pytest.raises(L.E3xError) as e:
         Elas3D((1291, 1291, 1291), 2)
     assert e.value.code == L.E3X_EINVAL
+
+
+def test_stop_rule_needs_a_handle_without_gpu():
+    lib = L.load()
+    assert lib.e3x_set_stop_rule(None, L.STOP_NIST_ABS) == L.E3X_EINVAL
+    assert b"NULL" in lib.e3x_last_error()

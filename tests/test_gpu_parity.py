@@ -257,6 +257,74 @@ def test_maxiter_and_history_prefix(ldemb):
     assert _rel(h, r_o["h"]) < 1e-10
 
 
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_nist_absolute_stop_rule(precision):
+    """E3X_STOP_NIST_ABS ends the PCG at the first iteration K whose unpreconditioned
+    gradient has sum(g^2) < 1e-8 * ns (NIST ELAS3D's gtest).  The oracle, run with tol = 0
+    for K and K - 1 iterations, must show the crossing there, and the iterates up to K are
+    those of the reference's rule.  Moduli in GPa, the units that test is scaled for."""
+    c11, c12, c44 = C11 * 1e-9, C12 * 1e-9, C44 * 1e-9
+    shape = (12, 12, 12)
+    ns = int(np.prod(shape))
+    pix, rod = voronoi_case(shape, 5, seed=410)
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    with Elas3D(shape, 6) as s:
+        s.ppixel(pix, rod)
+        s.femat(c11, c12, c44, e6)
+        with pytest.raises(L.E3xError) as e:
+            s.set_stop_rule(2)
+        assert e.value.code == L.E3X_EINVAL
+        s.set_stop_rule(L.STOP_NIST_ABS)
+        s.affine_guess()
+        _, gg0 = s.energy()
+        d = s.dembx(gg0, precision=precision)
+        u = s.get(L.FIELD_U)
+        gb = s.get(L.FIELD_GB)
+        # back to the default rule on the same handle: the reference's stop iteration
+        s.set_stop_rule(L.STOP_REFERENCE)
+        s.affine_guess()
+        s.energy()
+        d_ref = s.dembx(gg0, precision=precision)
+    K = int(d["lstep"][0])
+    assert d["exit"][0] == L.PCG_CONVERGED and 1 < K < 10000
+    rot = oc.rotate_stiffness(rod, c11, c12, c44)
+    dk_o = oc.dk(oc.cmod(6, rot, [0.0, 0.0]))
+    b_o, c_o = oc.femat_rhs(shape, pix, dk_o, e6)
+    u0 = oc.affine(shape, e6)
+    g0, _ = oc.energy(shape, pix, dk_o, b_o, c_o, u0)
+    ks = range(K - 1, K + 1) if precision == L.PREC_FP64 else range(max(K - 2, 1), K + 3)
+    runs = {k: oc.dembx(shape, pix, dk_o, u0, g0, oc.sumsq(g0), tol=0.0, ldemb=k) for k in ks}
+    gsq = {k: oc.sumsq(r["gb"]) for k, r in runs.items()}
+    gtest = 1e-8 * ns
+    # the reference's rule on the same handle; in these units its absolute hAh floor
+    # (|hAh| < 1e-14, f90:1899) may end the loop before rel_res < tol does
+    r_ref = oc.dembx(shape, pix, dk_o, u0, g0, oc.sumsq(g0), tol=oc.TOL_F32)
+    assert d_ref["exit"][0] == r_ref["status"]
+    if precision == L.PREC_FP64:
+        assert abs(int(d_ref["lstep"][0]) - r_ref["lstep"]) <= max(1, int(0.01 * r_ref["lstep"]))
+        assert gsq[K] < gtest <= gsq[K - 1]
+        assert _rel(u, runs[K]["u"]) < 1e-10
+        assert abs(oc.sumsq(gb) - gsq[K]) < 1e-8 * gsq[K]
+    else:
+        # FP32 gradient: the crossing may move by up to two iterations either way
+        assert min(gsq.values()) < gtest
+        assert _rel(u, runs[K]["u"]) < 1e-5
+
+
+def test_nist_absolute_stop_rule_per_rhs():
+    """Three right-hand sides under E3X_STOP_NIST_ABS stop where their single solves do."""
+    c11, c12, c44 = C11 * 1e-9, C12 * 1e-9, C44 * 1e-9
+    shape = (14, 12, 10)
+    pix, rod = voronoi_case(shape, 4, seed=420)
+    e3 = np.array([[oc.AML_F32, 0, 0, 0, 0, 0], [0, 0, 0, 0, 2e-3, 0], [0, 5e-4, -5e-4, 0, 0, 0]])
+    many = solve(pix, rod, shape, 5, c11, c12, c44, e3, stop_rule=L.STOP_NIST_ABS)
+    for r in range(3):
+        one = solve(pix, rod, shape, 5, c11, c12, c44, e3[r], stop_rule=L.STOP_NIST_ABS)
+        assert many.exit[r] == one.exit[0] == L.PCG_CONVERGED
+        assert many.lstep[r] == one.lstep[0]
+        assert abs(many.utot[r] - one.utot[0]) < 1e-12 * abs(one.utot[0])
+
+
 @pytest.mark.parametrize("ldemb,replace_every", [(15, 0), (16, 0), (17, 0), (17, 8)])
 def test_mixed_steps_reach_u_once_in_order(ldemb, replace_every):
     """The mixed form stopped by the iteration limit after an odd or an even number of
