@@ -336,6 +336,15 @@ int e3x_set_stop_rule(e3x_solver *h, int rule)
     return E3X_OK;
 }
 
+int e3x_set_pcg_variant(e3x_solver *h, int variant)
+{
+    if (int rc = check_idle(h)) return rc;
+    if (variant != E3X_PCG_REFERENCE && variant != E3X_PCG_SWEEP && variant != E3X_PCG_PLAIN_CG)
+        return fail(E3X_EINVAL, "variant must be E3X_PCG_REFERENCE, E3X_PCG_SWEEP or E3X_PCG_PLAIN_CG");
+    S(h).pcg_variant = variant;
+    return E3X_OK;
+}
+
 int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const double *gg0, float tol, int ldemb,
                     int kkk)
 {
@@ -358,7 +367,8 @@ int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const doubl
     E3X_HIP(hipEventRecord(s.ev[0], s.stream));
     // block-Jacobi inverse, built once per femat (f90:1498-1520)
     const bool need32 = mixed && !s.have_precond32;
-    if (!s.have_precond || kkk == 1 || need32) {
+    const bool kind_changed = s.precond_identity != (s.pcg_variant == E3X_PCG_PLAIN_CG);
+    if (!s.have_precond || kkk == 1 || need32 || kind_changed) {
         E3X_HIP(launch_precond(s, mixed || s.have_precond32));
         s.have_precond = true;
         if (mixed) s.have_precond32 = true;

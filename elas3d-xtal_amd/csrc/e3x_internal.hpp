@@ -33,9 +33,11 @@ struct PcgState {
     int exit_flag;    // e3x_pcg_exit
     int reset;        // this iteration reset h = z (f90:1961-1981)
     int reset_count;
-    int run_dir;      // the stopping iteration still updates h (f90:2007 follows 1996)
+    int run_dir;      // the loop ended after this iteration's direction update (f90:2007 follows 1996)
     int ldemb;
     int stop_rule;    // e3x_stop_rule
+    int no_reset;     // E3X_PCG_SWEEP, E3X_PCG_PLAIN_CG: no divergence reset
+    int stop_first;   // E3X_PCG_SWEEP, E3X_PCG_PLAIN_CG: the stop test precedes the direction update
     int pad;
     double gsq_tol;   // E3X_STOP_NIST_ABS: stop when the sum of g^2 falls below this
     double gsq;       // the last iteration's sum of g^2 (E3X_STOP_NIST_ABS)
@@ -100,6 +102,8 @@ struct Solver {
     // launches no blocks for them
     uint32_t pcg_known_stopped = 0;
     int stop_rule = 0;  // e3x_stop_rule of the next PCG
+    int pcg_variant = 0;  // e3x_pcg_variant of the next PCG
+    bool precond_identity = false;  // minv (minv32) hold the identity (E3X_PCG_PLAIN_CG)
 
     bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
     double last_precond_ms = 0, last_pcg_ms = 0;

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *parti
     const double rel = sqrt(S.gg / S.gg0);
     S.rel_res = rel;
     hist[(int64_t)r * S.ldemb + it - 1] = rel;
-    if (rel > 10.0 * S.rel_res_prev && it > 1) {
+    if (S.no_reset == 0 && rel > 10.0 * S.rel_res_prev && it > 1) {
         S.reset = 1;
         S.rel_res_min = rel;
         S.reset_count += 1;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *parti
     if (S.stop_rule == 1 ? gsq[0] < S.gsq_tol : rel < S.tol) {
         S.stop_iter = it;
         S.exit_flag = 1;  // E3X_PCG_CONVERGED
-        S.run_dir = 1;
+        S.run_dir = S.stop_first == 0;  // the variants exit before the direction update
     } else if (it >= S.ldemb) {
         S.stop_iter = it;
         S.exit_flag = 3;  // E3X_PCG_MAXITER
@@ -257,13 +257,14 @@ __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *parti
 
 // After the loop, per right-hand side that ended at iteration K by convergence or the
 // iteration limit: u -= lambda h_K unless iteration K already did (f90:1914-1918), and
-// the direction update that follows the stop test (f90:1961-1996, 2007) into h.  After
-// a curvature break (f90:1899-1902) only h_K is moved into h.  h_K sits in h (K odd) or
-// h_alt (K even).
+// the direction update that follows the stop test (f90:1961-1996, 2007) into h; where
+// the loop ended before that update (run_dir 0: the sweep and plain-CG loops on
+// convergence, A_2_0:1518, cg:1050) h_K itself.  After a curvature break (f90:1899-1902)
+// only h_K is moved into h.  h_K sits in h (K odd) or h_alt (K even).
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_pcg_finalize(PcgArgs<T> A, T *h_alt, int replace_every)
 {
-    int mode[NR];   // 0 nothing, 1 step + direction, 2 copy h_K
+    int mode[NR];   // 0 nothing, 1 step + direction, 2 copy h_K, 3 step + copy h_K
     bool dou[NR], rst[NR];
     double lam[NR], gam[NR];
     bool any = false;
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_finalize(PcgArgs<T> A, T *h_alt,
         const int K = S.stop_iter;
         mode[r] = 0;
         if (K >= 1) {
-            if (S.exit_flag == 1 || S.exit_flag == 3) mode[r] = 1;
+            if (S.exit_flag == 1 || S.exit_flag == 3) mode[r] = S.run_dir ? 1 : 3;
             else if (!(K & 1)) mode[r] = 2;
         }
         dou[r] = !(replace_every > 0 && K % replace_every == 0);
@@ -295,6 +296,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_finalize(PcgArgs<T> A, T *h_alt,
                 const T hv = hk[o];
                 if (mode[r] == 2) {
                     A.h[o] = hv;
+                } else if (mode[r] == 3) {
+                    if (dou[r]) A.u[o] = A.u[o] - lam[r] * (double)hv;
+                    if (hk != A.h) A.h[o] = hv;
                 } else {
                     if (dou[r]) A.u[o] = A.u[o] - lam[r] * (double)hv;
                     A.h[o] = rst[r] ? A.z[o] : (T)((double)A.z[o] + gam[r] * (double)hv);
@@ -434,6 +438,8 @@ hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
         hs[r].tol = (double)tol;
         hs[r].ldemb = ldemb;
         hs[r].stop_rule = s.stop_rule;
+        hs[r].no_reset = s.pcg_variant != 0;    // E3X_PCG_SWEEP, E3X_PCG_PLAIN_CG
+        hs[r].stop_first = s.pcg_variant != 0;
         hs[r].gsq_tol = 1e-8 * (double)s.ns;  // NIST ELAS3D: gtest = 1e-8 * (number of voxels)
     }
     hipError_t e = hipMemcpyAsync(s.st, hs, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream);

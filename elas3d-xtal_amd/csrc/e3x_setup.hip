@@ -317,13 +317,21 @@ __global__ __launch_bounds__(kBlock) void k_affine(int nx, int ny, int nz, int n
 
 // ---------------------------------------------------------------------------
 // Block-Jacobi preconditioner (f90:1498-1520): M = sum over the 8 voxels of the node
-// of dk(p, a, :, a, :), inverted by cofactors (invert3x3 f90:2046-2147).
+// of dk(p, a, :, a, :), inverted by cofactors (invert3x3 f90:2046-2147).  identity: the
+// plain-CG loop's M^-1 = I (z = 1 g0 + 0 g1 + 0 g2 = g exactly for finite g).
 __global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int64_t ns,
                                                     const uint16_t *pix, const double *dk,
-                                                    double *minv, float *minv32)
+                                                    double *minv, float *minv32, int identity)
 {
     const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (m >= ns) return;
+    if (identity) {
+        for (int q = 0; q < 6; q++) {
+            minv[6 * m + q] = q < 3 ? 1.0 : 0.0;
+            if (minv32) minv32[6 * m + q] = q < 3 ? 1.0f : 0.0f;
+        }
+        return;
+    }
     const int i = (int)(m % nx), j = (int)((m / nx) % ny), k = (int)(m / ((int64_t)nx * ny));
     double M[3][3];
     for (int a = 0; a < 8; a++) {
@@ -398,7 +406,9 @@ hipError_t launch_affine(Solver &s)
 hipError_t launch_precond(Solver &s, bool want32)
 {
     hipLaunchKernelGGL(k_precond, dim3((unsigned)vector_blocks(s)), dim3(kBlock), 0, s.stream, s.nx, s.ny,
-                       s.nz, s.ns, s.pix, s.dk, s.minv, want32 ? s.minv32 : nullptr);
+                       s.nz, s.ns, s.pix, s.dk, s.minv, want32 ? s.minv32 : nullptr,
+                       s.pcg_variant == 2 ? 1 : 0);
+    s.precond_identity = s.pcg_variant == 2;
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@ PCG_RUNNING, PCG_CONVERGED, PCG_HAH_BREAK, PCG_MAXITER = range(4)
 FIELD_U, FIELD_GB, FIELD_B, FIELD_H = range(4)
 PREC_FP64, PREC_MIXED = 0, 1
 STOP_REFERENCE, STOP_NIST_ABS = 0, 1
+PCG_REFERENCE, PCG_SWEEP, PCG_PLAIN_CG = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -48,6 +49,7 @@ SIGNATURES = {
     "e3x_dembx_ex": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_float, C.c_int, C.c_int, _ip, _dp,
                                _dp, _ip, _ip]),
     "e3x_set_stop_rule": (C.c_int, [_vp, C.c_int]),
+    "e3x_set_pcg_variant": (C.c_int, [_vp, C.c_int]),
     "e3x_dembx_begin": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_float, C.c_int, C.c_int]),
     "e3x_dembx_advance": (C.c_int, [_vp, C.c_int, _ip]),
     "e3x_dembx_end": (C.c_int, [_vp, _ip, _dp, _dp, _ip, _ip]),

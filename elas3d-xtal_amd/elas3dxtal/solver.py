@@ -44,6 +44,7 @@ class Elas3D:
         self.strain6 = np.zeros((self.nrhs, 6))
         self.C = np.zeros(self.nrhs)
         self.stop_rule = L.STOP_REFERENCE
+        self.pcg_variant = L.PCG_REFERENCE
 
     # -- lifetime -------------------------------------------------------------------
     def close(self):
@@ -116,6 +117,14 @@ class Elas3D:
         unpreconditioned gradient (tol is then unused)."""
         L.check(self._lib.e3x_set_stop_rule(self._h, rule))
         self.stop_rule = rule
+
+    def set_pcg_variant(self, variant):
+        """PCG loop of the following runs (e3x_set_pcg_variant): L.PCG_REFERENCE, the loop
+        of src/elas3dxtal_pcg.f90 (default); L.PCG_SWEEP, the anisotropy-sweep programs'
+        loop (no divergence reset, stop test before the direction update, A_2_0:1518);
+        L.PCG_PLAIN_CG, the plain-CG programs' loop (no preconditioner either, cg:915)."""
+        L.check(self._lib.e3x_set_pcg_variant(self._h, variant))
+        self.pcg_variant = variant
 
     def dembx(self, gg0, tol=TOL_DEFAULT, ldemb=LDEMB_DEFAULT, kkk=1, precision=L.PREC_FP64,
               replace_every=50, want_history=True):
@@ -212,11 +221,12 @@ class SolveResult:
 
 def solve(pix, orientation, shape, nphase, c11, c12, c44, strain6, phasemod=(0.0, 0.0),
           flag_m=1, tol=TOL_DEFAULT, ldemb=LDEMB_DEFAULT, precision=L.PREC_FP64,
-          replace_every=50, device=0, keep=False, stop_rule=L.STOP_REFERENCE):
+          replace_every=50, device=0, keep=False, stop_rule=L.STOP_REFERENCE, variant=L.PCG_REFERENCE):
     """The main program's sequence (f90:326-540) for one microstructure.
 
     strain6 may be (6,) or (nrhs, 6); with nrhs > 1 the load cases are solved in one
-    batched PCG.  stop_rule selects the PCG's stop test (Elas3D.set_stop_rule).
+    batched PCG.  stop_rule selects the PCG's stop test (Elas3D.set_stop_rule), variant
+    its loop (Elas3D.set_pcg_variant).
     Returns SolveResult (and the live Elas3D if keep=True).
     """
     e = np.atleast_2d(np.asarray(strain6, float))
@@ -224,6 +234,8 @@ def solve(pix, orientation, shape, nphase, c11, c12, c44, strain6, phasemod=(0.0
     try:
         if stop_rule != L.STOP_REFERENCE:
             s.set_stop_rule(stop_rule)
+        if variant != L.PCG_REFERENCE:
+            s.set_pcg_variant(variant)
         s.ppixel(pix, orientation)
         prob = s.assig()
         cval = s.femat(c11, c12, c44, e, phasemod=phasemod, flag_m=flag_m)

@@ -14,6 +14,7 @@
       integer(c_int), parameter :: E3X_FIELD_U = 0, E3X_FIELD_GB = 1, E3X_FIELD_B = 2, E3X_FIELD_H = 3
       integer(c_int), parameter :: E3X_PREC_FP64 = 0, E3X_PREC_MIXED = 1
       integer(c_int), parameter :: E3X_STOP_REFERENCE = 0, E3X_STOP_NIST_ABS = 1
+      integer(c_int), parameter :: E3X_PCG_REFERENCE = 0, E3X_PCG_SWEEP = 1, E3X_PCG_PLAIN_CG = 2
 
       interface
         function e3x_last_error() bind(C, name="e3x_last_error") result(p)
@@ -85,6 +86,14 @@
           import :: c_int, c_ptr
           type(c_ptr), value :: h
           integer(c_int), value :: rule
+        end function
+
+        ! loop of the following dembx calls: src (f90:1356), the anisotropy-sweep
+        ! programs' (A_2_0:1045) or the plain-CG programs' (cg:915) dembx_OpenMP
+        integer(c_int) function e3x_set_pcg_variant(h, variant) bind(C, name="e3x_set_pcg_variant")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+          integer(c_int), value :: variant
         end function
 
         ! dembx_OpenMP (f90:1356-2043), with a precision mode

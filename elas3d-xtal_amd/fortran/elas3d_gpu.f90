@@ -30,13 +30,13 @@
       integer :: kmax = 1, ldemb = 10000
       real :: tol = 1.0d-8                          ! `real` (f90:148)
       integer :: precision = E3X_PREC_FP64, replace_every = 50, device = 0
-      integer :: stop_rule = E3X_STOP_REFERENCE
+      integer :: stop_rule = E3X_STOP_REFERENCE, pcg_variant = E3X_PCG_REFERENCE
       character(len=512) :: input_file = 'input_structure_poly.h5'
       character(len=512) :: output_dir = '.'
       namelist /elas3d/ nx, ny, nz, n_grains, c11_local, c12_local, c44_local, e1, nu1, &
                         k_matrix, g_matrix, flag_m, aml, aml_exx, aml_eyy, aml_ezz, aml_exz, &
                         aml_eyz, aml_exy, kmax, ldemb, tol, precision, replace_every, device, &
-                        stop_rule, input_file, output_dir
+                        stop_rule, pcg_variant, input_file, output_dir
 
       ! ---- state ------------------------------------------------------------------
       type(c_ptr) :: h = c_null_ptr
@@ -112,6 +112,7 @@
                      'ppixel_hdf5', io=.true.)
       call e3x_check(e3x_create(nx, ny, nz, nphase, 1, device, h), 'e3x_create')
       call e3x_check(e3x_set_stop_rule(h, stop_rule), 'e3x_set_stop_rule')
+      call e3x_check(e3x_set_pcg_variant(h, pcg_variant), 'e3x_set_pcg_variant')
       call e3x_check(e3x_set_microstructure(h, pix, orientation), 'e3x_set_microstructure')
       print *, 'Unique phases in pix:', minval(pix), maxval(pix)
       do i = 1, nphase

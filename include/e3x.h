@@ -57,6 +57,22 @@ enum e3x_stop_rule {
                                not for moduli in Pa */
 };
 
+/* PCG loop variant (e3x_set_pcg_variant): the loop of one of the reference's programs,
+ * all on the same stencil, reductions and stop rule. */
+enum e3x_pcg_variant {
+    E3X_PCG_REFERENCE = 0, /* src/elas3dxtal_pcg.f90 dembx_OpenMP (f90:1356-2043): block-
+                              Jacobi preconditioner, 10x divergence reset (f90:1961-1981),
+                              direction update before the stop test (f90:1996, 2007).
+                              Default. */
+    E3X_PCG_SWEEP = 1,     /* applications/sec3_2/A_x_y/elas3dxtal_pcg.f90 dembx_OpenMP
+                              (A_2_0:1045-1558): block Jacobi, no reset, stop test before
+                              the direction update (A_2_0:1518-1526) */
+    E3X_PCG_PLAIN_CG = 2   /* applications/sec3_1/cg/n_N/elas3dxtal_cg.f90 dembx_OpenMP
+                              (cg:915-1074; the serial variant's loop is the same): no
+                              preconditioner (z = g, h = gb at entry, cg:949), no reset,
+                              stop test before the direction update (cg:1050-1056) */
+};
+
 /* Fields reachable with e3x_get_field / e3x_set_field. */
 enum e3x_field {
     E3X_FIELD_U = 0,  /* displacement u(ns,3)          (e3d%u)  */
@@ -165,6 +181,14 @@ int e3x_dembx_ex(e3x_solver *s, int precision, int replace_every, const double *
  * the reference, whose code has only the relative rule (f90:2007).
  */
 int e3x_set_stop_rule(e3x_solver *s, int rule);
+
+/*
+ * PCG loop of the handle's following e3x_dembx / e3x_dembx_ex / e3x_dembx_begin calls
+ * (e3x_pcg_variant; E3X_PCG_REFERENCE until set): dembx_OpenMP of src/elas3dxtal_pcg.f90
+ * (f90:1356), of the anisotropy-sweep programs (A_2_0:1045) or of the plain-CG programs
+ * (cg:915).  With E3X_PCG_PLAIN_CG the preconditioner is the identity.
+ */
+int e3x_set_pcg_variant(e3x_solver *s, int variant);
 
 /*
  * e3x_dembx_ex in three steps, so that one host thread can keep several handles'

@@ -569,6 +569,19 @@ int oc_dembx(int nx, int ny, int nz, const int32_t *pix, const double *dk, doubl
              double *gb, double *h_io, double gg0, float tol, int ldemb, int *lstep, double *gg_out,
              double *history, int *reset_count_out)
 {
+    return oc_dembx_variant(nx, ny, nz, pix, dk, u, gb, h_io, gg0, tol, ldemb, lstep, gg_out, history,
+                            reset_count_out, OC_PCG_REFERENCE);
+}
+
+/* The three loops differ only where marked: the preconditioner (plain CG: none, z = gb,
+ * h = gb at entry, cg:949), the divergence reset (src only) and the order of the stop
+ * test and the direction update (src: update first, f90:1996/2007; sweep and plain CG:
+ * test first, A_2_0:1518-1526, cg:1050-1056). */
+int oc_dembx_variant(int nx, int ny, int nz, const int32_t *pix, const double *dk, double *u,
+                     double *gb, double *h_io, double gg0, float tol, int ldemb, int *lstep,
+                     double *gg_out, double *history, int *reset_count_out, int variant)
+{
+    const int plain = variant == OC_PCG_PLAIN_CG, test_first = variant != OC_PCG_REFERENCE;
     const int64_t ns = (int64_t)nx * ny * nz;
     const double tol_d = (double)tol;
     int64_t nch = (ns + CHUNK - 1) / CHUNK;
@@ -578,8 +591,9 @@ int oc_dembx(int nx, int ny, int nz, const int32_t *pix, const double *dk, doubl
     double *ah = (double *)malloc(sizeof(double) * 3 * (size_t)ns);
     double *h = h_io ? h_io : (double *)malloc(sizeof(double) * 3 * (size_t)ns);
 
-    oc_precond(nx, ny, nz, pix, dk, minv);           /* kkk == 1 (f90:1498-1520) */
-    apply_minv(ns, minv, gb, z);                      /* f90:1527-1535 */
+    if (!plain) oc_precond(nx, ny, nz, pix, dk, minv);  /* kkk == 1 (f90:1498-1520) */
+    if (plain) memcpy(z, gb, sizeof(double) * 3 * (size_t)ns);  /* h = gb (cg:949) */
+    else apply_minv(ns, minv, gb, z);                 /* f90:1527-1535 */
     double gg = dot3(ns, z, gb, part);                /* f90:1541-1548 */
     memcpy(h, z, sizeof(double) * 3 * (size_t)ns);    /* f90:1554-1562 */
     double rel_res_prev = sqrt(gg / gg0), rel_res_min = rel_res_prev, rel_res = rel_res_prev;
@@ -597,12 +611,14 @@ int oc_dembx(int nx, int ny, int nz, const int32_t *pix, const double *dk, doubl
             u[q] = u[q] - lambda * h[q];
             gb[q] = gb[q] - lambda * ah[q];
         }
-        apply_minv(ns, minv, gb, z);                          /* f90:1927-1935 */
+        if (plain) memcpy(z, gb, sizeof(double) * 3 * (size_t)ns);
+        else apply_minv(ns, minv, gb, z);                     /* f90:1927-1935 */
         double gglast = gg;
-        gg = dot3(ns, z, gb, part);                           /* f90:1938-1946 */
+        gg = dot3(ns, z, gb, part);                           /* f90:1938-1946; cg:1040-1046 */
         rel_res = sqrt(gg / gg0);                             /* f90:1949 */
         if (history) history[L - 1] = rel_res;
-        if (rel_res > 10.0 * rel_res_prev && L > 1) {         /* f90:1961-1981 */
+        if (test_first && rel_res < tol_d) { status = OC_CONVERGED; break; } /* A_2_0:1518, cg:1050 */
+        if (!test_first && rel_res > 10.0 * rel_res_prev && L > 1) { /* f90:1961-1981 */
             memcpy(h, z, sizeof(double) * 3 * (size_t)ns);
             rel_res_min = rel_res;
             reset_count = reset_count + 1;
@@ -613,7 +629,7 @@ int oc_dembx(int nx, int ny, int nz, const int32_t *pix, const double *dk, doubl
             if (rel_res < rel_res_min) rel_res_min = rel_res;
         }
         rel_res_prev = rel_res;                               /* f90:2004 */
-        if (rel_res < tol_d) { status = OC_CONVERGED; break; } /* f90:2007 */
+        if (rel_res < tol_d) { status = OC_CONVERGED; break; } /* f90:2007 (reached by src only) */
     }
     *lstep = L;
     *gg_out = gg;

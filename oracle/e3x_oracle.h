@@ -74,6 +74,14 @@ int oc_dembx(int nx, int ny, int nz, const int32_t *pix, const double *dk, doubl
              double *gb, double *h, double gg0, float tol, int ldemb, int *lstep, double *gg,
              double *history, int *reset_count);
 
+/* The PCG loops of the reference's programs (include/e3x.h e3x_pcg_variant):
+ * src/elas3dxtal_pcg.f90 (f90:1356-2043), applications/sec3_2/A_x_y/elas3dxtal_pcg.f90
+ * (A_2_0:1045-1558) and applications/sec3_1/cg/n_N/elas3dxtal_cg.f90 (cg:915-1074). */
+enum { OC_PCG_REFERENCE = 0, OC_PCG_SWEEP = 1, OC_PCG_PLAIN_CG = 2 };
+int oc_dembx_variant(int nx, int ny, int nz, const int32_t *pix, const double *dk, double *u,
+                     double *gb, double *h, double gg0, float tol, int ldemb, int *lstep, double *gg,
+                     double *history, int *reset_count, int variant);
+
 /* f90:2149-2408: volume-averaged stress6 (str*) and engineering strain6 (s*),
  * order xx,yy,zz,xz,yz,xy. */
 void oc_stress(int nx, int ny, int nz, const int32_t *pix, const double *cmod,

@@ -199,18 +199,23 @@ def block_jacobi_dense(kg, ns):
     return out
 
 
-def pcg_reference_dense(kg, minv, u, g, gg0, tol, ldemb):
+def pcg_reference_dense(kg, minv, u, g, gg0, tol, ldemb, reset=True, test_first=False):
     """The PCG of dembx_OpenMP written from its text (src/elas3dxtal_pcg.f90:1527-2007)
     over a dense matrix: z = M^-1 g, h = z; per step hAh (break if |hAh| < 1e-14),
     lambda = gg/hAh, u -= lambda h, g -= lambda Ah, z = M^-1 g, rel_res = sqrt(z.g/gg0);
     if rel_res > 10 rel_res_prev (step > 1) h = z and count a reset, else
     h = z + (gg/gglast) h; then stop when rel_res < tol.
+    The other programs' loops: minv None is the plain CG of elas3dxtal_cg.f90 (z = g,
+    cg:949-1056); reset=False drops the reset and test_first=True stops before the
+    direction update (sweep A_2_0:1514-1526 and plain CG cg:1049-1056).
     Returns dict(u, g, h, history, lstep, resets, status) with status 1 converged,
     2 curvature break, 3 iteration limit."""
     ns = kg.shape[0] // 3
     u, g = u.copy(), g.copy()
 
     def prec(v):
+        if minv is None:
+            return v.copy()
         w = v.reshape(3, ns).T                     # (ns, 3)
         return np.einsum("mij,mj->mi", minv, w).T.ravel()
 
@@ -232,7 +237,10 @@ def pcg_reference_dense(kg, minv, u, g, gg0, tol, ldemb):
         gglast, gg = gg, float(z @ g)
         rel = np.sqrt(gg / gg0)
         hist.append(rel)
-        if rel > 10.0 * rel_prev and step > 1:
+        if test_first and rel < tol:
+            status = 1
+            break
+        if reset and rel > 10.0 * rel_prev and step > 1:
             h = z.copy()
             resets += 1
         else:

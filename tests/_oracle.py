@@ -47,6 +47,8 @@ def _load():
         "oc_precond": (v, [i, i, i, _ip, _dp, _dp]),
         "oc_dembx": (i, [i, i, i, _ip, _dp, _dp, _dp, _dp, d, f, i, C.POINTER(C.c_int),
                          C.POINTER(C.c_double), _dp, C.POINTER(C.c_int)]),
+        "oc_dembx_variant": (i, [i, i, i, _ip, _dp, _dp, _dp, _dp, d, f, i, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_double), _dp, C.POINTER(C.c_int), i]),
         "oc_stress": (v, [i, i, i, _ip, _dp, _dp, _dp, _dp, _dp]),
         "oc_fullfield": (v, [i, i, i, _ip, _dp, _dp, _dp, _dp, _dp]),
         "oc_solve": (i, [i, i, i, i, i, _ip, _dp, d, d, d, _dp, _dp, f, i, C.c_void_p,
@@ -161,15 +163,18 @@ def precond(shape, pix, dkm):
     return out.reshape(-1, 3, 3)
 
 
-def dembx(shape, pix, dkm, u, gb, gg0, tol=TOL_F32, ldemb=10000):
+PCG_REFERENCE, PCG_SWEEP, PCG_PLAIN_CG = 0, 1, 2
+
+
+def dembx(shape, pix, dkm, u, gb, gg0, tol=TOL_F32, ldemb=10000, variant=PCG_REFERENCE):
     nx, ny, nz = shape
     u = _f64(u).copy()
     gb = _f64(gb).copy()
     h = np.zeros_like(u)
     hist = np.zeros(max(ldemb, 1))
     L, gg, rc = C.c_int(), C.c_double(), C.c_int()
-    st = lib.oc_dembx(nx, ny, nz, _i32(pix), _f64(dkm).ravel(), u, gb, h, gg0, tol, ldemb,
-                      C.byref(L), C.byref(gg), hist, C.byref(rc))
+    st = lib.oc_dembx_variant(nx, ny, nz, _i32(pix), _f64(dkm).ravel(), u, gb, h, gg0, tol, ldemb,
+                              C.byref(L), C.byref(gg), hist, C.byref(rc), variant)
     return dict(status=st, lstep=L.value, gg=gg.value, u=u, gb=gb, h=h,
                 history=hist[:L.value].copy(), reset_count=rc.value)
 

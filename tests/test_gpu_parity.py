@@ -325,6 +325,88 @@ def test_nist_absolute_stop_rule_per_rhs():
         assert abs(many.utot[r] - one.utot[0]) < 1e-12 * abs(one.utot[0])
 
 
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_isotropic_matrix_void_inclusion_matches_oracle(precision):
+    """flag_m = 0, the Eshelby programs' mode (applications/sec3_1/pcg/n_N/
+    elas3dxtal_pcg.f90:59-79, 190-198): one isotropic matrix phase with the Voigt bulk
+    and shear moduli of the cubic constants, and a centred spherical void (E1 = nu1 = 0)
+    under ezz."""
+    shape = (24, 22, 20)
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r2 = (i - nx / 2 + 0.5) ** 2 + (j - ny / 2 + 0.5) ** 2 + (k - nz / 2 + 0.5) ** 2
+    pix = np.where(r2.ravel() <= 5.0 ** 2, 2, 1).astype(np.int32)
+    rod = np.zeros((1, 3))
+    k0, g0 = (C11 + 2 * C12) / 3.0, (C11 - C12 + 3 * C44) / 5.0
+    pm = (k0, g0, 0.0, 0.0)
+    e6 = [0, 0, oc.AML_F32, 0, 0, 0]
+    ro = oc.solve(shape, 2, pix, rod, C11, C12, C44, e6, phasemod=pm, flag_m=0)
+    res = solve(pix, rod, shape, 2, C11, C12, C44, e6, phasemod=pm, flag_m=0, precision=precision)
+    assert res.exit[0] == L.PCG_CONVERGED == ro["status"]
+    if precision == L.PREC_FP64:
+        assert abs(int(res.lstep[0]) - ro["lstep"]) <= max(1, int(0.01 * ro["lstep"]))
+        assert abs(res.utot[0] - ro["utot"]) < 1e-10 * abs(ro["utot"])
+        assert _rel(res.stress6[0], ro["stress6"]) < 1e-10
+    else:
+        assert abs(int(res.lstep[0]) - ro["lstep"]) <= max(2, int(0.05 * ro["lstep"]))
+        assert abs(res.utot[0] - ro["utot"]) < 1e-6 * abs(ro["utot"])
+        assert _rel(res.stress6[0], ro["stress6"]) < 1e-5
+    # the void carries no load: the mean szz falls below the matrix's c33 * ezz
+    assert 0 < res.stress6[0][2] < (k0 + 4 * g0 / 3) * oc.AML_F32
+
+
+@pytest.mark.parametrize("variant", [L.PCG_SWEEP, L.PCG_PLAIN_CG])
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_pcg_variants_match_oracle(variant, precision):
+    """The anisotropy-sweep and plain-CG loops against the oracle's, for stop tolerances
+    that end them after an odd and an even number of iterations (the final direction is
+    then h_K, held in either direction buffer); then the same handle back on the
+    reference loop (the block-Jacobi inverse is rebuilt after the plain CG's identity)."""
+    shape = (13, 10, 7)
+    pix, rod = voronoi_case(shape, 6, seed=310)
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    _, dk_o = oracle_tables(7, rod)
+    b_o, c_o = oc.femat_rhs(shape, pix, dk_o, e6)
+    u0 = oc.affine(shape, e6)
+    g0, _ = oc.energy(shape, pix, dk_o, b_o, c_o, u0)
+    parities = set()
+    with Elas3D(shape, 7) as s:
+        s.ppixel(pix, rod)
+        s.femat(C11, C12, C44, e6)
+        s.set_pcg_variant(variant)
+        for tol in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7, oc.TOL_F32):
+            r_o = oc.dembx(shape, pix, dk_o, u0, g0, oc.sumsq(g0), tol=tol, variant=variant)
+            s.affine_guess()
+            _, gg0 = s.energy()
+            d = s.dembx(gg0, tol=tol, precision=precision)
+            u, h = s.get(L.FIELD_U), s.get(L.FIELD_H)
+            K = int(d["lstep"][0])
+            assert d["exit"][0] == r_o["status"] == L.PCG_CONVERGED
+            assert d["reset_count"][0] == 0
+            if precision == L.PREC_FP64:
+                assert abs(K - r_o["lstep"]) <= max(1, int(0.01 * r_o["lstep"]))
+                if K == r_o["lstep"]:
+                    parities.add(K & 1)
+                    assert _rel(u, r_o["u"]) < 1e-10
+                    assert _rel(d["history"][0], r_o["history"]) < 1e-8
+                    assert _rel(h, r_o["h"]) < 1e-5
+            else:
+                parities.add(K & 1)
+                assert abs(K - r_o["lstep"]) <= max(2, int(0.05 * r_o["lstep"]))
+                assert _rel(u, r_o["u"]) < 1e-5
+        s.set_pcg_variant(L.PCG_REFERENCE)
+        s.affine_guess()
+        _, gg0 = s.energy()
+        d = s.dembx(gg0)
+        u = s.get(L.FIELD_U)
+    assert parities == {0, 1}
+    r_ref = oc.dembx(shape, pix, dk_o, u0, g0, oc.sumsq(g0))
+    assert d["exit"][0] == L.PCG_CONVERGED
+    assert abs(int(d["lstep"][0]) - r_ref["lstep"]) <= max(1, int(0.01 * r_ref["lstep"]))
+    if d["lstep"][0] == r_ref["lstep"]:
+        assert _rel(u, r_ref["u"]) < 1e-10
+
+
 @pytest.mark.parametrize("ldemb,replace_every", [(15, 0), (16, 0), (17, 0), (17, 8)])
 def test_mixed_steps_reach_u_once_in_order(ldemb, replace_every):
     """The mixed form stopped by the iteration limit after an odd or an even number of

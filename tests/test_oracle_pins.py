@@ -296,6 +296,62 @@ def test_pcg_divergence_reset_matches_dense_restatement():
     assert _rel(r["h"], d["h"]) < 1e-2       # h is of the size of the final residual
 
 
+def _pcg_case(shape, ngrain, seed):
+    pix, rod, cm, dk = _random_case(shape, ngrain, seed=seed, pore_frac=0.03)
+    e6 = [oc.AML_F32, 0, 0, 0, 0.4e-3, 0]
+    b, cval = oc.femat_rhs(shape, pix, dk, e6)
+    u0 = oc.affine(shape, e6)
+    g0, _ = oc.energy(shape, pix, dk, b, cval, u0)
+    return pix, dk, u0, g0, oc.sumsq(g0)
+
+
+@pytest.mark.parametrize("variant", [oc.PCG_SWEEP, oc.PCG_PLAIN_CG])
+def test_pcg_variants_match_dense_restatement(variant):
+    """The sweep and plain-CG loops of the oracle against the dense-matrix loop written
+    from those programs' text (no reset, stop test before the direction update; plain
+    CG without preconditioner)."""
+    shape = (5, 4, 4)
+    ns = int(np.prod(shape))
+    pix, dk, u0, g0, gg0 = _pcg_case(shape, 3, seed=52)
+    r = oc.dembx(shape, pix, dk, u0, g0, gg0, variant=variant)
+    kg = ind.assemble_dense(shape, pix, ind.dk_to_kmats(dk))
+    minv = None if variant == oc.PCG_PLAIN_CG else ind.block_jacobi_dense(kg, ns)
+    d = ind.pcg_reference_dense(kg, minv, u0, g0, gg0, oc.TOL_F32, 10000, reset=False, test_first=True)
+    assert r["status"] == oc.OC_CONVERGED and d["status"] == 1
+    assert r["lstep"] == d["lstep"] and r["reset_count"] == 0
+    assert _rel(r["history"], d["history"]) < 1e-8
+    assert _rel(r["u"], d["u"]) < 1e-10
+    assert _rel(r["h"], d["h"]) < 1e-4       # h is of the size of the final residual
+
+
+def test_plain_cg_residuals_are_orthogonal():
+    """Textbook CG (M = I, Fletcher-Reeves gamma): successive gradients are mutually
+    orthogonal.  A preconditioner left in, or a wrong gamma, breaks it."""
+    shape = (4, 4, 3)
+    pix, dk, u0, g0, gg0 = _pcg_case(shape, 3, seed=53)
+    gs = [g0] + [oc.dembx(shape, pix, dk, u0, g0, gg0, tol=0.0, ldemb=k, variant=oc.PCG_PLAIN_CG)["gb"]
+                 for k in range(1, 7)]
+    for i in range(len(gs)):
+        for j in range(i):
+            c = abs(gs[i] @ gs[j]) / (np.linalg.norm(gs[i]) * np.linalg.norm(gs[j]))
+            assert c < 1e-8, (i, j, c)
+
+
+def test_sweep_loop_is_the_reference_loop_without_its_last_direction():
+    """Without a reset the sweep loop takes the reference loop's steps; it stops before
+    the direction update, so its final h is the direction the reference loop holds after
+    K - 1 iterations."""
+    shape = (6, 5, 4)
+    pix, dk, u0, g0, gg0 = _pcg_case(shape, 4, seed=54)
+    src = oc.dembx(shape, pix, dk, u0, g0, gg0)
+    sw = oc.dembx(shape, pix, dk, u0, g0, gg0, variant=oc.PCG_SWEEP)
+    assert src["reset_count"] == 0 and src["lstep"] == sw["lstep"]
+    assert np.array_equal(src["u"], sw["u"]) and np.array_equal(src["history"], sw["history"])
+    km1 = oc.dembx(shape, pix, dk, u0, g0, gg0, tol=0.0, ldemb=sw["lstep"] - 1)
+    assert np.array_equal(sw["h"], km1["h"])
+    assert not np.array_equal(src["h"], sw["h"])
+
+
 def test_pcg_hah_break_on_all_pore_grid():
     shape = (3, 3, 3)
     ns = 27
