@@ -45,7 +45,7 @@ void release(Solver &s)
     (void)hipSetDevice(s.device);
     void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.perm, s.u, s.g, s.b, s.h, s.h_alt, s.z,
                    s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
-                   s.st, s.hist, s.results, s.post};
+                   s.st, s.hist, s.results, s.post, s.hot};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (s.results_host) (void)hipHostFree(s.results_host);
@@ -545,6 +545,38 @@ int e3x_stress_fullfield(e3x_solver *h, int rhs, double *stress, double *vm)
         E3X_HIP(hipMemcpyAsync(stress, dsf, sizeof(double) * 6 * (size_t)s.ns, hipMemcpyDeviceToHost, s.stream));
     if (vm) E3X_HIP(hipMemcpyAsync(vm, dvm, sizeof(double) * (size_t)s.ns, hipMemcpyDeviceToHost, s.stream));
     E3X_HIP(hipStreamSynchronize(s.stream));
+    return E3X_OK;
+}
+
+int e3x_hotspots(e3x_solver *h, int rhs, const double *vm, int nsolid, double sigma, double threshold_mpa,
+                 int npeaks, int64_t *index, double *value, int64_t *found)
+{
+    if (int rc = check_idle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.have_pix) return fail(E3X_ESTATE, "e3x_hotspots before e3x_set_microstructure");
+    if (!vm && !s.have_femat) return fail(E3X_ESTATE, "e3x_hotspots without vm before e3x_femat");
+    if (rhs < 0 || rhs >= s.nrhs) return fail(E3X_EINVAL, "rhs out of range");
+    if (!(sigma > 0.0 && sigma <= 1.5)) return fail(E3X_EINVAL, "sigma must be in (0, 1.5]");
+    if (npeaks < 0 || npeaks > kHotPeaksMax) return fail(E3X_EINVAL, "npeaks must be in [0, 64]");
+    if (npeaks > 0 && (!index || !value)) return fail(E3X_EINVAL, "index or value is NULL");
+    if (!std::isfinite(threshold_mpa)) return fail(E3X_EINVAL, "threshold_mpa must be finite");
+    E3X_HIP(hipSetDevice(s.device));
+    if (!s.post) E3X_HIP(dalloc(s, &s.post, 7 * s.ns));
+    if (!s.hot) E3X_HIP(dalloc(s, &s.hot, 2 + 2 * (int64_t)kHotPeaksMax));
+    double *dvm = s.post + 6 * s.ns;
+    if (vm) E3X_HIP(hipMemcpyAsync(dvm, vm, sizeof(double) * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
+    else E3X_HIP(launch_fullfield(s, rhs, nullptr, dvm));
+    HotResult res;
+    E3X_HIP(launch_hotspots(s, nsolid, sigma, (float)threshold_mpa, npeaks, res));
+    for (size_t k = 0; k < res.peaks.size(); k++) {
+        index[k] = res.peaks[k].m;
+        value[k] = (double)res.peaks[k].v;
+    }
+    for (int k = (int)res.peaks.size(); k < npeaks; k++) {
+        index[k] = -1;
+        value[k] = 0.0;
+    }
+    if (found) *found = res.found;
     return E3X_OK;
 }
 

@@ -78,6 +78,7 @@ struct Solver {
     float *h32_alt = nullptr;
     double *cconst = nullptr;                                  // C per rhs
     double *post = nullptr;                                    // full-field staging [7][ns]
+    double *hot = nullptr;   // hotspot pass: two counters, then kHotPeaksMax entries
 
     // reductions and PCG state
     double *partials = nullptr;
@@ -140,6 +141,19 @@ hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t s
 // one right-hand side's field: component planes [3][ns] <-> node-interleaved [ns][3]
 hipError_t launch_planes_to_nodes(const double *src, double *dst, int64_t ns, hipStream_t st);
 hipError_t launch_nodes_to_planes(const double *src, double *dst, int64_t ns, hipStream_t st);
+
+// e3x_hotspot.hip: regional maxima of the smoothed von Mises field in s.post's vm plane
+struct HotEntry {
+    float v;
+    int pad;
+    int64_t m;
+};
+constexpr int kHotPeaksMax = 64;
+struct HotResult {
+    int64_t found = 0;
+    std::vector<HotEntry> peaks;
+};
+hipError_t launch_hotspots(Solver &s, int nsolid, double sigma, float thr_mpa, int npeaks, HotResult &res);
 
 // e3x_api.hip: reuse a handle for another microstructure with nphase <= nphase_cap
 // phases (the serving loop's handle pool); the microstructure and tables must be

@@ -54,6 +54,8 @@ SIGNATURES = {
     "e3x_dembx_advance": (C.c_int, [_vp, C.c_int, _ip]),
     "e3x_dembx_end": (C.c_int, [_vp, _ip, _dp, _dp, _ip, _ip]),
     "e3x_stress": (C.c_int, [_vp, _dp, _dp]),
+    "e3x_hotspots": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
+                               C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int64)]),
     "e3x_effective_stiffness": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_double,
                                           C.c_int, C.c_int, C.c_float, C.c_int, _dp, _ip, _ip]),
     "e3x_stress_fullfield": (C.c_int, [_vp, C.c_int, _dp, _dp]),

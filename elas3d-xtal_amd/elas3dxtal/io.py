@@ -122,3 +122,30 @@ def read_fullfield(path, ns):
     pix, vm, st = np.empty(ns, np.int32), np.empty(ns), np.empty(6 * ns)
     _check(load().e3x_h5_read_fullfield(path.encode(), ns, _i(pix), _d(vm), _d(st)))
     return pix, vm, st.reshape(6, ns)
+
+
+def format_top_peaks(shape, index, value, xvec=None, yvec=None, zvec=None, nsave=3):
+    """top_peaks.txt of the reference's post-processing (applications/sec3_2/A_2_0/
+    elas3dxtal_postprocessing.m:223-243): the first min(3, n) peaks, value in MPa (the
+    single-precision smoothed field divided by 1e6 in single, .m:207) and the voxel
+    coordinates xvec(i), yvec(j), zvec(k) (default: the 0-based indices).  Raises, as the
+    script does (.m:117), when no peak was found."""
+    nx, ny, nz = shape
+    if len(index) == 0:
+        raise ValueError("No hotspots found exceeding the threshold!")
+    xs = np.arange(nx, dtype=float) if xvec is None else np.asarray(xvec, float)
+    ys = np.arange(ny, dtype=float) if yvec is None else np.asarray(yvec, float)
+    zs = np.arange(nz, dtype=float) if zvec is None else np.asarray(zvec, float)
+    n = min(nsave, len(index))
+    lines = [f"Top {n} Local Maxima (Von Mises Stress):"]
+    for k in range(n):
+        m = int(index[k])
+        i, j, kk = m % nx, (m // nx) % ny, m // (nx * ny)
+        mpa = float(np.float32(value[k]) / np.float32(1e6))
+        lines.append(f"Peak #{k + 1}: {mpa:.3f} MPa at ({xs[i]:.2f}, {ys[j]:.2f}, {zs[kk]:.2f})")
+    return "\n".join(lines) + "\n"
+
+
+def write_top_peaks(path, shape, index, value, xvec=None, yvec=None, zvec=None):
+    with open(path, "w") as f:
+        f.write(format_top_peaks(shape, index, value, xvec, yvec, zvec))

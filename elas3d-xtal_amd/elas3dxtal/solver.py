@@ -45,6 +45,7 @@ class Elas3D:
         self.C = np.zeros(self.nrhs)
         self.stop_rule = L.STOP_REFERENCE
         self.pcg_variant = L.PCG_REFERENCE
+        self._pix_max = 0
 
     # -- lifetime -------------------------------------------------------------------
     def close(self):
@@ -83,6 +84,7 @@ class Elas3D:
                 raise L.E3xError(L.E3X_EINVAL, f"orientation size {ori.size} does not match "
                                                f"3*(nphase-1) = {3 * (self.nphase - 1)}")
         L.check(self._lib.e3x_set_microstructure(self._h, L.i32ptr(pix), L.dptr(ori)))
+        self._pix_max = int(pix.max())
 
     def assig(self):
         prob = np.empty(self.nphase)
@@ -173,6 +175,28 @@ class Elas3D:
         sf, vm = np.zeros(6 * self.ns), np.zeros(self.ns)
         L.check(self._lib.e3x_stress_fullfield(self._h, rhs, L.dptr(sf), L.dptr(vm)))
         return sf.reshape(6, self.ns), vm
+
+    def hotspots(self, rhs=0, vm=None, nsolid=None, sigma=0.4, threshold_mpa=250.0, npeaks=3):
+        """Hotspots of the von Mises field as the reference's post-processing finds them
+        for top_peaks.txt (applications/sec3_2/A_2_0/elas3dxtal_postprocessing.m:86-243),
+        on the device (e3x_hotspots).  vm: a field in node order, or None for this
+        handle's own (right-hand side rhs).  nsolid None: the largest phase label, as the
+        scripts take it (.m:67).  Returns (index[k] 0-based node numbers, value[k] smoothed
+        von Mises in Pa, found)."""
+        if nsolid is None:
+            nsolid = int(self._pix_max)
+        vmv = None
+        if vm is not None:
+            vmv = np.ascontiguousarray(vm, dtype=np.float64).ravel()
+            if vmv.size != self.ns:
+                raise L.E3xError(L.E3X_EINVAL, "vm size must be ns")
+        idx = np.zeros(max(npeaks, 1), np.int64)
+        val = np.zeros(max(npeaks, 1))
+        found = C.c_int64()
+        L.check(self._lib.e3x_hotspots(self._h, rhs, L.dptr(vmv), nsolid, sigma, threshold_mpa, npeaks,
+                                       idx.ctypes.data_as(C.POINTER(C.c_int64)), L.dptr(val), C.byref(found)))
+        n = min(npeaks, found.value)
+        return idx[:n].copy(), val[:n].copy(), found.value
 
     # -- fields ---------------------------------------------------------------------
     def get(self, which, rhs=0):

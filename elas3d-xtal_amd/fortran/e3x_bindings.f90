@@ -109,6 +109,19 @@
           real(c_double) :: gg(*), history(*)
         end function
 
+        ! the MATLAB post-processing's hotspots for top_peaks.txt
+        ! (applications/sec3_2/A_2_0/elas3dxtal_postprocessing.m:86-243); vm may be
+        ! c_null_ptr for the handle's own field of right-hand side rhs
+        integer(c_int) function e3x_hotspots(h, rhs, vm, nsolid, sigma, threshold_mpa, npeaks, &
+                                             idx, val, found) bind(C, name="e3x_hotspots")
+          import :: c_int, c_ptr, c_double, c_int64_t
+          type(c_ptr), value :: h, vm
+          integer(c_int), value :: rhs, nsolid, npeaks
+          real(c_double), value :: sigma, threshold_mpa
+          integer(c_int64_t) :: idx(*), found
+          real(c_double) :: val(*)
+        end function
+
         ! stress_OpenMP (f90:2149-2408)
         integer(c_int) function e3x_stress(h, stress6, strain6) bind(C, name="e3x_stress")
           import :: c_int, c_ptr, c_double

@@ -191,6 +191,33 @@ int e3x_set_stop_rule(e3x_solver *s, int rule);
 int e3x_set_pcg_variant(e3x_solver *s, int variant);
 
 /*
+ * Hotspots of a von Mises field as the reference's MATLAB post-processing finds them for
+ * top_peaks.txt (applications/sec3_2/A_2_0/elas3dxtal_postprocessing.m:86-243; the sec3_3
+ * scripts take the same steps):
+ *   1. solid voxels: phase label (1-based) <= nsolid.  The scripts set nsolid = max(pix)
+ *      (.m:67), which counts pores as solid;
+ *   2. masked normalised Gaussian smoothing with zero padding (the grid is not wrapped):
+ *      s = G*(vm on the solid) / G*(solid), G = exp(-d^2 / (2 sigma^2)) for |d| <=
+ *      ceil(2 sigma) per axis, normalised (imgaussfilt3).  The numerator is double; the
+ *      denominator and the quotient are single precision, as the script's double ./
+ *      single gives.  A denominator below 1e-6 counts as 1.  s = 0 off the solid
+ *      (.m:86-97);
+ *   3. regional maxima of s with 26-connectivity (imregionalmax, .m:206): connected sets
+ *      of equal value whose outside neighbours are all lower (every member counts);
+ *   4. those with s / 1e6 >= threshold_mpa, in single precision (.m:207-208), sorted by
+ *      value, descending, ties by linear index (.m:215).
+ * vm: ns values in node order (the /vm dataset of fullfield_poly.h5), or NULL for the
+ * handle's own full field of right-hand side rhs (computed on the device as
+ * e3x_stress_fullfield does; needs e3x_femat).  Writes the first npeaks (<= 64) peaks:
+ * index[k] = i + nx (j + ny k), 0-based, and value[k] = s there in Pa (single precision
+ * widened), -1 and 0 past the last one.  *found (may be NULL) is the number of regional
+ * maxima at or above the threshold.  sigma in (0, 1.5].  No counterpart in the
+ * reference's Fortran: it is its MATLAB post-processing, on the device.
+ */
+int e3x_hotspots(e3x_solver *s, int rhs, const double *vm, int nsolid, double sigma, double threshold_mpa,
+                 int npeaks, int64_t *index, double *value, int64_t *found);
+
+/*
  * e3x_dembx_ex in three steps, so that one host thread can keep several handles'
  * solves in flight on the GPU at once (each handle has its own stream); no counterpart
  * in the reference.  e3x_dembx_begin validates, builds the preconditioner and enqueues

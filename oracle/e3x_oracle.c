@@ -786,3 +786,134 @@ int oc_solve(int nx, int ny, int nz, int nphase, int flag_m, const int32_t *pix,
     free(rot); free(cmod); free(dk); free(b); free(h);
     return 0;
 }
+
+/* ---------------------------------------------------------------- postprocessing.m:86-243 */
+typedef struct {
+    float v;
+    int64_t m;
+} oc_peak;
+
+static int oc_peak_cmp(const void *a, const void *b)
+{
+    const oc_peak *x = (const oc_peak *)a, *y = (const oc_peak *)b;
+    if (x->v != y->v) return x->v > y->v ? -1 : 1;  /* sort(..., 'descend') (.m:215) */
+    return x->m < y->m ? -1 : (x->m > y->m ? 1 : 0); /* stable: find() order */
+}
+
+/* One separable zero-padded pass of a (2r+1) kernel along axis `ax` (imgaussfilt3 with
+ * 'Padding', 0 filters each dimension in turn). */
+static void conv_axis_d(int nx, int ny, int nz, int ax, int r, const double *w, const double *in, double *out)
+{
+    const int n[3] = {nx, ny, nz};
+    const int64_t stride[3] = {1, nx, (int64_t)nx * ny};
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int c[3] = {i, j, k};
+                const int64_t m = i + (int64_t)nx * (j + (int64_t)ny * k);
+                double acc = 0.0;
+                for (int d = -r; d <= r; d++) {
+                    const int p = c[ax] + d;
+                    if (p < 0 || p >= n[ax]) continue;
+                    acc += w[d < 0 ? -d : d] * in[m + d * stride[ax]];
+                }
+                out[m] = acc;
+            }
+}
+
+static void conv_axis_f(int nx, int ny, int nz, int ax, int r, const float *w, const float *in, float *out)
+{
+    const int n[3] = {nx, ny, nz};
+    const int64_t stride[3] = {1, nx, (int64_t)nx * ny};
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int c[3] = {i, j, k};
+                const int64_t m = i + (int64_t)nx * (j + (int64_t)ny * k);
+                float acc = 0.0f;
+                for (int d = -r; d <= r; d++) {
+                    const int p = c[ax] + d;
+                    if (p < 0 || p >= n[ax]) continue;
+                    acc += w[d < 0 ? -d : d] * in[m + d * stride[ax]];
+                }
+                out[m] = acc;
+            }
+}
+
+int64_t oc_hotspots(int nx, int ny, int nz, const int32_t *pix, const double *vm, int nsolid, double sigma,
+                    double thr_mpa, int npeaks, int64_t *index, float *value, float *sm_out)
+{
+    const int64_t ns = (int64_t)nx * ny * nz;
+    const int r = (int)ceil(2.0 * sigma);
+    double w[16], tot = 0.0;
+    float wf[16];
+    for (int d = -r; d <= r; d++) tot += exp(-(double)d * d / (2.0 * sigma * sigma));
+    for (int d = 0; d <= r; d++) {
+        w[d] = exp(-(double)d * d / (2.0 * sigma * sigma)) / tot;
+        wf[d] = (float)w[d];
+    }
+    double *a = (double *)malloc(sizeof(double) * (size_t)ns), *b = (double *)malloc(sizeof(double) * (size_t)ns);
+    float *fa = (float *)malloc(sizeof(float) * (size_t)ns), *fb = (float *)malloc(sizeof(float) * (size_t)ns);
+    float *sm = (float *)malloc(sizeof(float) * (size_t)ns);
+    for (int64_t m = 0; m < ns; m++) {                  /* .m:86-88, 92 */
+        const int sol = pix[m] <= nsolid;
+        a[m] = sol ? vm[m] : 0.0;
+        fa[m] = sol ? 1.0f : 0.0f;
+    }
+    conv_axis_d(nx, ny, nz, 0, r, w, a, b);              /* .m:91 */
+    conv_axis_d(nx, ny, nz, 1, r, w, b, a);
+    conv_axis_d(nx, ny, nz, 2, r, w, a, b);
+    conv_axis_f(nx, ny, nz, 0, r, wf, fa, fb);           /* .m:92 */
+    conv_axis_f(nx, ny, nz, 1, r, wf, fb, fa);
+    conv_axis_f(nx, ny, nz, 2, r, wf, fa, fb);
+    for (int64_t m = 0; m < ns; m++) {
+        float den = fb[m];
+        if (den < 1e-6f) den = 1.0f;                     /* .m:93 */
+        sm[m] = pix[m] <= nsolid ? (float)b[m] / den : 0.0f; /* .m:96-97 */
+    }
+    /* imregionalmax (.m:206): flood-fill each equal-valued component */
+    uint8_t *seen = (uint8_t *)calloc((size_t)ns, 1);
+    int64_t *stack = (int64_t *)malloc(sizeof(int64_t) * (size_t)ns);
+    int64_t *comp = (int64_t *)malloc(sizeof(int64_t) * (size_t)ns);
+    oc_peak *pk = (oc_peak *)malloc(sizeof(oc_peak) * (size_t)ns);
+    int64_t npk = 0;
+    for (int64_t s0 = 0; s0 < ns; s0++) {
+        if (seen[s0]) continue;
+        const float v = sm[s0];
+        int64_t top = 0, nc = 0;
+        int is_max = 1;
+        stack[top++] = s0;
+        seen[s0] = 1;
+        while (top > 0) {
+            const int64_t m = stack[--top];
+            comp[nc++] = m;
+            const int i = (int)(m % nx), j = (int)((m / nx) % ny), k = (int)(m / ((int64_t)nx * ny));
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int ii = i + dx, jj = j + dy, kk = k + dz;
+                        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kk < 0 || kk >= nz) continue;
+                        const int64_t q = ii + (int64_t)nx * (jj + (int64_t)ny * kk);
+                        if (sm[q] > v) is_max = 0;
+                        if (sm[q] == v && !seen[q]) {
+                            seen[q] = 1;
+                            stack[top++] = q;
+                        }
+                    }
+        }
+        if (!is_max || !(v / 1e6f >= (float)thr_mpa)) continue; /* .m:207-208 */
+        for (int64_t c = 0; c < nc; c++) {
+            pk[npk].v = v;
+            pk[npk].m = comp[c];
+            npk++;
+        }
+    }
+    qsort(pk, (size_t)npk, sizeof(oc_peak), oc_peak_cmp);
+    for (int p = 0; p < npeaks; p++) {
+        index[p] = p < npk ? pk[p].m : -1;
+        value[p] = p < npk ? pk[p].v : 0.0f;
+    }
+    if (sm_out) memcpy(sm_out, sm, sizeof(float) * (size_t)ns);
+    free(a); free(b); free(fa); free(fb); free(sm); free(seen); free(stack); free(comp); free(pk);
+    return npk;
+}

@@ -107,6 +107,15 @@ int oc_solve(int nx, int ny, int nz, int nphase, int flag_m, const int32_t *pix,
              const double strain6[6], float tol, int ldemb, double *u_out, double *gb_out,
              double *history, oc_result *res);
 
+/* Hotspots of a von Mises field: applications/sec3_2/A_2_0/elas3dxtal_postprocessing.m
+ * :86-215 (solid = label <= nsolid; masked Gaussian smoothing, zero padding, separable
+ * passes x, y, z, numerator double and denominator single; regional maxima by flood fill
+ * of equal-valued 26-connected components; threshold s/1e6 >= thr_mpa in single;
+ * sorted by value descending, ties by index).  Writes the smoothed field to sm if
+ * non-NULL and the first npeaks peaks; returns the number of peaks found. */
+int64_t oc_hotspots(int nx, int ny, int nz, const int32_t *pix, const double *vm, int nsolid, double sigma,
+                    double thr_mpa, int npeaks, int64_t *index, float *value, float *sm);
+
 /* Threads used by the OpenMP loops (0 = library default). */
 void oc_set_threads(int n);
 int oc_get_threads(void);

@@ -50,3 +50,18 @@ def ceff_from_cases(stress6, strain6):
     S = np.asarray(stress6, float).T
     E = np.asarray(strain6, float).T
     return S @ np.linalg.inv(E)
+
+
+def hot_fixture():
+    """A 9x8x7 field at 100 MPa with: a two-voxel plateau of 300 MPa (both are regional
+    maxima), a three-voxel plateau of 280 MPa touching a 290 MPa voxel (not maxima; the
+    290 is), and a lone 260 MPa voxel at the grid's corner (a maximum: no wrap)."""
+    shape = (9, 8, 7)
+    nx, ny, nz = shape
+    vm = np.full(int(np.prod(shape)), 100e6)
+    at = lambda i, j, k: i + nx * (j + ny * k)
+    vm[at(2, 2, 2)] = vm[at(3, 2, 2)] = 300e6
+    vm[at(6, 5, 4)] = vm[at(6, 6, 4)] = vm[at(6, 6, 5)] = 280e6
+    vm[at(7, 6, 5)] = 290e6
+    vm[at(0, 0, 0)] = 260e6
+    return shape, vm, at

@@ -49,6 +49,8 @@ def _load():
                          C.POINTER(C.c_double), _dp, C.POINTER(C.c_int)]),
         "oc_dembx_variant": (i, [i, i, i, _ip, _dp, _dp, _dp, _dp, d, f, i, C.POINTER(C.c_int),
                                  C.POINTER(C.c_double), _dp, C.POINTER(C.c_int), i]),
+        "oc_hotspots": (C.c_int64, [i, i, i, _ip, _dp, i, d, d, i, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "oc_stress": (v, [i, i, i, _ip, _dp, _dp, _dp, _dp, _dp]),
         "oc_fullfield": (v, [i, i, i, _ip, _dp, _dp, _dp, _dp, _dp]),
         "oc_solve": (i, [i, i, i, i, i, _ip, _dp, d, d, d, _dp, _dp, f, i, C.c_void_p,
@@ -216,3 +218,17 @@ def solve(shape, nphase, pix, rod, c11, c12, c44, strain6, phasemod=(0.0, 0.0), 
     if want_fields:
         out["u"], out["gb"] = u, gb
     return out
+
+
+def hotspots(shape, pix, vm, nsolid, sigma=0.4, thr_mpa=250.0, npeaks=3, want_smooth=False):
+    """-> (index[npeaks], value[npeaks] float32, found[, smoothed field float32])."""
+    nx, ny, nz = shape
+    ns = nx * ny * nz
+    idx = np.zeros(npeaks, np.int64)
+    val = np.zeros(npeaks, np.float32)
+    sm = np.zeros(ns, np.float32) if want_smooth else None
+    fp = C.POINTER(C.c_float)
+    n = lib.oc_hotspots(nx, ny, nz, _i32(pix), _f64(vm), nsolid, sigma, thr_mpa, npeaks,
+                        idx.ctypes.data_as(C.POINTER(C.c_int64)), val.ctypes.data_as(fp),
+                        sm.ctypes.data_as(fp) if want_smooth else None)
+    return (idx, val, int(n), sm) if want_smooth else (idx, val, int(n))

@@ -11,6 +11,7 @@ from scipy.spatial.transform import Rotation
 
 import _indep as ind
 import _oracle as oc
+from _cases import hot_fixture
 
 C11, C12, C44 = 246.5e9, 147.3e9, 124.7e9
 
@@ -456,3 +457,74 @@ def test_effective_stiffness_between_reuss_and_voigt():
     assert np.min(np.linalg.eigvalsh(voigt - sym)) > -1e-6 * scale
     assert np.min(np.linalg.eigvalsh(sym - reuss)) > -1e-6 * scale
     assert np.max(np.linalg.eigvalsh(voigt - sym)) > 1e-3 * scale      # strictly inside
+
+
+# ---------------------------------------------------------------- hotspots (postprocessing.m:86-243)
+def test_hotspot_plateaus_and_borders():
+    shape, vm, at = hot_fixture()
+    pix = np.ones(vm.size, np.int32)
+    # sigma 0.05: the off-centre weights are exp(-200), so the smoothing leaves the field
+    # unchanged and the plateaus stay exact
+    idx, val, n = oc.hotspots(shape, pix, vm, 1, sigma=0.05, thr_mpa=250.0, npeaks=5)
+    assert n == 4
+    assert list(idx[:4]) == [at(2, 2, 2), at(3, 2, 2), at(7, 6, 5), at(0, 0, 0)]
+    assert list(val[:4]) == [np.float32(300e6)] * 2 + [np.float32(290e6), np.float32(260e6)]
+    assert idx[4] == -1
+    # a constant field is one regional maximum of every voxel; ties by index
+    idx, _, n = oc.hotspots(shape, pix, np.full(vm.size, 300e6), 1, sigma=0.05, thr_mpa=250.0)
+    assert n == vm.size and list(idx) == [0, 1, 2]
+
+
+def test_hotspot_smoothing_equals_scipy_masked_gaussian():
+    """Masked normalised smoothing against scipy.ndimage.gaussian_filter (zero padding,
+    radius ceil(2 sigma)), in double: the oracle's single-precision quotient agrees to
+    float rounding; pores (label > nsolid) read 0."""
+    from scipy.ndimage import gaussian_filter
+    shape = (11, 9, 8)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(60)
+    vm = rng.uniform(50e6, 400e6, int(np.prod(shape)))
+    pix = rng.integers(1, 4, vm.size).astype(np.int32)
+    pix[rng.random(vm.size) < 0.1] = 4
+    for sigma in (0.4, 0.9, 1.3):
+        _, _, _, sm = oc.hotspots(shape, pix, vm, 3, sigma=sigma, thr_mpa=250.0, want_smooth=True)
+        solid = (pix <= 3).reshape(nz, ny, nx)
+        v3 = vm.reshape(nz, ny, nx)
+        kw = dict(mode="constant", cval=0.0, truncate=np.ceil(2 * sigma) / sigma)
+        num = gaussian_filter(np.where(solid, v3, 0.0), sigma, **kw)
+        den = gaussian_filter(solid.astype(float), sigma, **kw)
+        want = np.where(solid, num / np.where(den < 1e-6, 1.0, den), 0.0).ravel()
+        assert np.max(np.abs(sm - want)) < 3e-7 * np.max(want)
+        assert np.all(sm[pix > 3] == 0.0)
+
+
+def test_hotspot_maxima_equal_maximum_filter_on_a_tie_free_field():
+    from scipy.ndimage import maximum_filter
+    shape = (12, 10, 9)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(61)
+    vm = rng.uniform(0.0, 500e6, int(np.prod(shape)))
+    pix = np.ones(vm.size, np.int32)
+    idx, val, n = oc.hotspots(shape, pix, vm, 1, sigma=0.4, thr_mpa=250.0, npeaks=8, want_smooth=True)[:3]
+    sm = oc.hotspots(shape, pix, vm, 1, sigma=0.4, thr_mpa=250.0, want_smooth=True)[3]
+    s3 = sm.reshape(nz, ny, nx)
+    mx = maximum_filter(s3, size=3, mode="constant", cval=-np.inf)
+    cand = np.flatnonzero(((s3 == mx) & (s3 / np.float32(1e6) >= np.float32(250.0))).ravel())
+    assert len(np.unique(sm)) == sm.size                 # no ties: the filter test is exact
+    assert n == cand.size
+    order = cand[np.argsort(-sm[cand], kind="stable")]
+    assert list(idx) == list(order[:8])
+    assert np.array_equal(val, sm[order[:8]])
+
+
+def test_top_peaks_text_format():
+    from elas3dxtal.io import format_top_peaks
+    shape = (4, 3, 2)
+    txt = format_top_peaks(shape, [5, 17], [np.float32(301.2345e6), np.float32(250.0e6)],
+                           xvec=[-1.5, -0.5, 0.5, 1.5], yvec=[0, 10, 20], zvec=[7.126, 8.0])
+    # node 5 = (1, 1, 0), node 17 = (1, 1, 1)
+    assert txt == ("Top 2 Local Maxima (Von Mises Stress):\n"
+                   "Peak #1: 301.234 MPa at (-0.50, 10.00, 7.13)\n"
+                   "Peak #2: 250.000 MPa at (-0.50, 10.00, 8.00)\n")
+    with pytest.raises(ValueError):
+        format_top_peaks(shape, [], [])
