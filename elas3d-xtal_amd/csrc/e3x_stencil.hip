@@ -426,7 +426,39 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
             }
             __syncthreads();
         }
-    perm[(int64_t)bid * kSB + t] = (PermT)(key[t] & 0xFFFull);
+    // Within each run of equal keys (same class and phases) any order serves: deal the run's
+    // nodes out by their LDS bank residue (halo index mod 32), round by round, so that the
+    // 16- and 32-lane groups that read LDS together hit distinct banks where the run allows
+    // it.  Key: run start, rank of the node among the run's nodes of its residue, residue.
+    static_assert(kSB <= 256, "the run key holds 8-bit positions");
+    const unsigned long long mine = key[t];
+    int start = t;
+    while (start > 0 && (key[start - 1] >> 12) == (mine >> 12)) start--;
+    auto residue = [](int node) {
+        const int w = node >> 6, l = node & 63;
+        const int x = wave_x(w) + (l & 3), y = wave_y(w) + ((l >> 2) & 3), z = wave_z(w) + (l >> 4);
+        return ((z + 1) * (HX * HY) + (y + 1) * HX + (x + 1)) & 31;
+    };
+    const int node = (int)(mine & 0xFFFull), res = residue(node);
+    int rank = 0;
+    for (int i = start; i < t; i++) rank += residue((int)(key[i] & 0xFFFull)) == res;
+    __syncthreads();
+    key[t] = ((unsigned long long)start << 24) | ((unsigned long long)rank << 16) | ((unsigned long long)res << 8) |
+             (unsigned long long)node;
+    __syncthreads();
+    for (int k = 2; k <= kSB; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int i = t, p = t ^ j;
+            if (p > i) {
+                const unsigned long long a = key[i], b = key[p];
+                if (((i & k) == 0) == (a > b)) {
+                    key[i] = b;
+                    key[p] = a;
+                }
+            }
+            __syncthreads();
+        }
+    perm[(int64_t)bid * kSB + t] = (PermT)(key[t] & 0xFFull);
 }
 
 template <int NR>
