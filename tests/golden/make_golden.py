@@ -1,0 +1,79 @@
+"""Writes the golden fixtures of this directory from the CPU oracle (oracle/, a
+restatement of src/elas3dxtal_pcg.f90 pinned by tests/test_oracle_pins.py).
+
+The reference ships no golden outputs, and it cannot be built in this image.  These
+files freeze the oracle's outputs on three small seeded cases, so that every later test
+run checks against the same numbers: the GPU tests against the files directly, and a
+CPU test checks that the oracle still reproduces them.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import _oracle as oc                      # noqa: E402
+from _cases import RESET_CASE, voronoi_case  # noqa: E402
+
+C11, C12, C44 = 246.5e9, 147.3e9, 124.7e9
+
+
+def case_poly():
+    """10x9x8 grid, 5 Voronoi grains + 3 % single-voxel pores, exx (the main program's
+    sequence, f90:326-540), with u and the full field (f90:2639-2833)."""
+    shape = (10, 9, 8)
+    pix, rod = voronoi_case(shape, 5, seed=900, pore_frac=0.03)
+    e6 = np.array([oc.AML_F32, 0, 0, 0, 0, 0])
+    r = oc.solve(shape, 6, pix, rod, C11, C12, C44, e6)
+    cm = oc.cmod(6, oc.rotate_stiffness(rod, C11, C12, C44), [0.0, 0.0])
+    sf, vm = oc.fullfield(shape, pix, cm, e6, r["u"])
+    return dict(shape=np.array(shape), nphase=6, pix=pix.astype(np.int32), rod=rod, strain6=e6,
+                phasemod=np.zeros(2), flag_m=1, lstep=r["lstep"], status=r["status"],
+                reset_count=r["reset_count"], utot0=r["utot0"], gg0=r["gg0"], utot=r["utot"], gg=r["gg"],
+                stress6=r["stress6"], strain_avg6=r["strain6"], history=r["history"], u=r["u"],
+                stress_field=sf.ravel(), vm=vm)
+
+
+def case_reset():
+    """The indefinite case whose PCG resets its direction (f90:1961-1981)."""
+    rc = RESET_CASE
+    shape = rc["shape"]
+    pix, rod = voronoi_case(shape, rc["ngrain"], rc["seed"], rc["pore_frac"])
+    e6 = np.array([oc.AML_F32, 0, 0, 0, 0, 0])
+    r = oc.solve(shape, rc["ngrain"] + 1, pix, rod, C11, C12, C44, e6, phasemod=rc["phasemod"])
+    return dict(shape=np.array(shape), nphase=rc["ngrain"] + 1, pix=pix.astype(np.int32), rod=rod, strain6=e6,
+                phasemod=np.array(rc["phasemod"]), flag_m=1, lstep=r["lstep"], status=r["status"],
+                reset_count=r["reset_count"], utot0=r["utot0"], gg0=r["gg0"], utot=r["utot"], gg=r["gg"],
+                stress6=r["stress6"], strain_avg6=r["strain6"], history=r["history"], u=r["u"])
+
+
+def case_void():
+    """flag_m = 0 (applications/sec3_1/pcg): isotropic matrix with the Voigt moduli of the
+    cubic constants and a centred spherical void, ezz."""
+    shape = (12, 12, 10)
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r2 = (i - nx / 2 + 0.5) ** 2 + (j - ny / 2 + 0.5) ** 2 + (k - nz / 2 + 0.5) ** 2
+    pix = np.where(r2.ravel() <= 3.0 ** 2, 2, 1).astype(np.int32)
+    pm = np.array([(C11 + 2 * C12) / 3.0, (C11 - C12 + 3 * C44) / 5.0, 0.0, 0.0])
+    e6 = np.array([0, 0, oc.AML_F32, 0, 0, 0])
+    r = oc.solve(shape, 2, pix, np.zeros((1, 3)), C11, C12, C44, e6, phasemod=pm, flag_m=0)
+    return dict(shape=np.array(shape), nphase=2, pix=pix, rod=np.zeros((1, 3)), strain6=e6, phasemod=pm,
+                flag_m=0, lstep=r["lstep"], status=r["status"], reset_count=r["reset_count"], utot0=r["utot0"],
+                gg0=r["gg0"], utot=r["utot"], gg=r["gg"], stress6=r["stress6"], strain_avg6=r["strain6"],
+                history=r["history"], u=r["u"])
+
+
+CASES = {"poly_10x9x8": case_poly, "reset_6x6x6": case_reset, "void_12x12x10": case_void}
+
+if __name__ == "__main__":
+    oc.lib.oc_set_threads(1)   # one thread: the oracle's sums in one fixed order
+    for name, fn in CASES.items():
+        d = fn()
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **{k: np.asarray(v) for k, v in d.items()})
+        print(name, "lstep", int(d["lstep"]), "status", int(d["status"]), os.path.getsize(path), "bytes")

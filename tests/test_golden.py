@@ -1,0 +1,76 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py from the
+oracle): the oracle must still reproduce them (CPU), and the GPU library must match them
+(marked gpu) with the parity bars of the GPU tests: FP64 fields, energies and averages to
+1e-10 relative, the iteration count within 1 %.  On the indefinite reset case the
+tolerances are those of its oracle pin, since summation order there grows to ~1e-6."""
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as oc
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NAMES = ["poly_10x9x8", "reset_6x6x6", "void_12x12x10"]
+C11, C12, C44 = 246.5e9, 147.3e9, 124.7e9
+
+
+def _load(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_reproduces_golden(name):
+    g = _load(name)
+    shape = tuple(int(v) for v in g["shape"])
+    oc.lib.oc_set_threads(1)
+    try:
+        r = oc.solve(shape, int(g["nphase"]), g["pix"], g["rod"], C11, C12, C44, g["strain6"],
+                     phasemod=g["phasemod"], flag_m=int(g["flag_m"]))
+    finally:
+        oc.lib.oc_set_threads(0)
+    assert (r["lstep"], r["status"], r["reset_count"]) == (int(g["lstep"]), int(g["status"]), int(g["reset_count"]))
+    for k in ("utot0", "gg0", "utot", "gg"):
+        assert abs(r[k] - float(g[k])) <= 1e-13 * abs(float(g[k])), k
+    assert _rel(r["stress6"], g["stress6"]) < 1e-13
+    assert _rel(r["history"], g["history"]) < 1e-12
+    assert _rel(r["u"], g["u"]) < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+def test_gpu_matches_golden(name):
+    from elas3dxtal import _lib as L
+    from elas3dxtal.solver import solve
+    g = _load(name)
+    shape = tuple(int(v) for v in g["shape"])
+    res, s = solve(g["pix"], g["rod"], shape, int(g["nphase"]), C11, C12, C44, g["strain6"],
+                   phasemod=g["phasemod"], flag_m=int(g["flag_m"]), keep=True)
+    try:
+        u = s.get(L.FIELD_U)
+        sf, vm = s.stress_fullfield(0) if "vm" in g else (None, None)
+    finally:
+        s.close()
+    reset = name.startswith("reset")
+    assert res.exit[0] == int(g["status"]) == L.PCG_CONVERGED
+    assert res.reset_count[0] == int(g["reset_count"])
+    lstep = int(g["lstep"])
+    assert abs(int(res.lstep[0]) - lstep) <= max(1, int(0.01 * lstep))
+    assert abs(res.utot0[0] - float(g["utot0"])) < 1e-12 * abs(float(g["utot0"]))
+    if res.lstep[0] != lstep:
+        return
+    tol = 1e-6 if reset else 1e-10
+    assert abs(res.utot[0] - float(g["utot"])) < (1e-8 if reset else 1e-10) * abs(float(g["utot"]))
+    assert _rel(u, g["u"]) < tol
+    assert _rel(res.history[0], g["history"]) < (1e-5 if reset else 1e-8)
+    assert _rel(res.stress6[0], g["stress6"]) < tol
+    assert _rel(res.strain6[0], g["strain_avg6"]) < 1e-12
+    if vm is not None:
+        assert _rel(sf.ravel(), g["stress_field"]) < 1e-10
+        assert _rel(vm, g["vm"]) < 1e-10
