@@ -281,16 +281,31 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
         }
     }
     if (__ballot(mlane)) {
+        // Octants in pairs (a, a ^ 6), a = 0..3: corner a ^ 6 is the one opposite a, and the
+        // element matrix is symmetric under inversion through the voxel centre
+        // (dk[P][a^6][i][b][j] = dk[P][a][i][b^6][j], the elasticity tensor being even), so
+        // the second octant of a pair reads the first one's coefficient rows with its node
+        // blocks taken in b ^ 6 order.  Half the table is read, and each slice twice in a
+        // row: the second pass finds it in the scalar cache.
 #pragma unroll 1
-        for (int a = 0; a < 8; a++) {
-            // voxel a of the node spans nodes (PX(b)-PX(a), PY(b)-PY(a), PZ(b)-PZ(a)), b = 0..7
-            const T *hb = hr - (PZ(a) * (HX * HY) + PY(a) * HX + PX(a));
+        for (int k = 0; k < 8; k++) {
+            const int a = k >> 1, oct = (k & 1) ? (a ^ 6) : a;
+            // voxel oct of the node spans nodes (PX(b)-PX(oct), ...), b = 0..7
+            const T *hb = hr - (PZ(oct) * (HX * HY) + PY(oct) * HX + PX(oct));
             T hv[24];
+            if (k & 1) {
 #pragma unroll
-            for (int b = 0; b < 8; b++)
+                for (int b = 0; b < 8; b++)
 #pragma unroll
-                for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
-            const int pa = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
+                    for (int n = 0; n < 3; n++)
+                        hv[b * 3 + n] = hb[n * HN + PZ(b ^ 6) * (HX * HY) + PY(b ^ 6) * HX + PX(b ^ 6)];
+            } else {
+#pragma unroll
+                for (int b = 0; b < 8; b++)
+#pragma unroll
+                    for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
+            }
+            const int pa = pc[-(PZ(oct) * (VX * VY) + PY(oct) * VX + PX(oct))];
             bool pend = mlane;
             while (true) {
                 const unsigned long long pm = __ballot(pend);
