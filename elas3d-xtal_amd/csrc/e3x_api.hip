@@ -408,9 +408,7 @@ int e3x_dembx_advance(e3x_solver *h, int block, int *done)
             const int n = std::min(R.chunk, R.ldemb - R.it + 1);
             for (int t = 0; t < n; t++) {
                 const int itt = R.it + t;
-                const bool replace = R.replace_every > 0 && itt % R.replace_every == 0;
-                const bool prev_replace = R.replace_every > 0 && itt > 1 && (itt - 1) % R.replace_every == 0;
-                E3X_HIP(launch_pcg_iter(s, R.mixed, itt, replace, prev_replace));
+                E3X_HIP(launch_pcg_iter(s, R.mixed, itt, R.mixed ? R.replace_every : 0));
             }
             R.it += n;
             // the stop_iter of every rhs into the pinned slot of this chunk
@@ -705,14 +703,16 @@ int e3x_time_apply(e3x_solver *h, int precision, int reps, double *avg_ms)
         saved[r] = s.st_host[r];
         s.st_host[r].stop_iter = 0;
         s.st_host[r].lambda = 0.0;  // u stays as it is
+        s.st_host[r].lambda_prev = 0.0;
         s.st_host[r].gamma = 0.0;   // h = z
         s.st_host[r].reset = 0;
     }
     E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
     if (mixed) E3X_HIP(launch_to_f32(s.h, s.h32, (int64_t)s.nrhs * 3 * s.ns, s.stream));
-    E3X_HIP(launch_apply_only(s, mixed));  // warm-up
+    // the PCG's applies alternate between none and two u steps: time them in turn
+    E3X_HIP(launch_apply_only(s, mixed, 2));  // warm-up
     E3X_HIP(hipEventRecord(s.ev[0], s.stream));
-    for (int i = 0; i < reps; i++) E3X_HIP(launch_apply_only(s, mixed));
+    for (int i = 0; i < reps; i++) E3X_HIP(launch_apply_only(s, mixed, (i & 1) ? 2 : 0));
     E3X_HIP(hipEventRecord(s.ev[1], s.stream));
     E3X_HIP(hipEventSynchronize(s.ev[1]));
     float ms = 0;

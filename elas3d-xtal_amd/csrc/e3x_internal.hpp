@@ -29,6 +29,7 @@ constexpr int kStenHalf = 13 * 9 + 6;
 // dembx_OpenMP, f90:1440-1445).
 struct PcgState {
     double gg, gg0, hah, lambda, gamma, rel_res, rel_res_prev, rel_res_min, tol;
+    double lambda_prev;  // lambda of the iteration before (the u steps go two at a time)
     int stop_iter;    // 0 while running, else the iteration at which the loop ended
     int exit_flag;    // e3x_pcg_exit
     int reset;        // this iteration reset h = z (f90:1961-1981)
@@ -42,6 +43,26 @@ struct PcgState {
     double gsq_tol;   // E3X_STOP_NIST_ABS: stop when the sum of g^2 falls below this
     double gsq;       // the last iteration's sum of g^2 (E3X_STOP_NIST_ABS)
 };
+
+// The u steps u -= lambda_k h_k (f90:1914-1918) reach u two at a time: the fused apply of
+// iteration it applies the two pending steps it-2 and it-1 when two are pending and none
+// otherwise, which saves the u read and write of every other apply; a true-gradient
+// replacement at iteration R (mixed form, every re iterations) brings u up to step R.
+// Each step is still applied alone, in order, so u is the same value as with one step per
+// iteration.  R0: the last replacement before iteration it (0 without any).
+__host__ __device__ inline int u_last_replace(int it, int re) { return re > 0 ? ((it - 1) / re) * re : 0; }
+// steps the fused apply of iteration it applies (0 or 2)
+__host__ __device__ inline int u_steps_of_apply(int it, int re)
+{
+    const int d = it - u_last_replace(it, re);
+    return (d >= 3 && (d & 1)) ? 2 : 0;
+}
+// the last step in u once the apply of iteration it has run
+__host__ __device__ inline int u_done_after_apply(int it, int re)
+{
+    const int r0 = u_last_replace(it, re), d = it - r0;
+    return r0 + 2 * ((d - 1) / 2);
+}
 
 // kApplyFused: the PCG apply of iterations >= 2, which first forms this iteration's
 // direction h = z + gamma h_prev (h = z on a reset) and takes the previous step into u
@@ -120,17 +141,17 @@ hipError_t launch_precond(Solver &s, bool want32);
 // e3x_stencil.hip
 hipError_t launch_energy(Solver &s);                          // results[0..2*nrhs)
 hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K x (FP64)
-// PCG apply of iteration it (fused direction / u step for it >= 2; fuse_u false when
-// the previous iteration already moved u), then hAh and lambda
-hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u);
-hipError_t launch_apply_only(Solver &s, bool mixed);          // fused apply with the current state
+// PCG apply of iteration it (fused direction, and for it >= 2 the u steps usteps = 0 or
+// 2 of u_steps_of_apply), then hAh and lambda
+hipError_t launch_apply_hah(Solver &s, bool mixed, int it, int usteps);
+hipError_t launch_apply_only(Solver &s, bool mixed, int usteps);  // fused apply with the current state
 hipError_t launch_resid(Solver &s);                           // g32 = K u + b in FP64
 hipError_t launch_tile_order(Solver &s);                      // perm from pix
 // e3x_pcg.hip
 hipError_t launch_hah_finish(Solver &s, int nparts, int it);
 hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
                            const double *gg0_host);
-hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace, bool prev_replace);
+hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, int replace_every);
 // after the loop: the step still owed to u and the final direction, into h (h32)
 hipError_t launch_pcg_finalize(Solver &s, bool mixed, int replace_every);
 // e3x_post.hip

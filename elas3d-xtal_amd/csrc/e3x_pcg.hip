@@ -3,14 +3,14 @@
 // One iteration = four launches on the solver's stream, all gated on device-resident
 // per-rhs stop state (so the host can enqueue iterations ahead and stop launching
 // when it sees every right-hand side finished):
-//   stencil  (it >= 2) h = z + gamma h_prev or z (f90:1961-1996) and u -= lambda h_prev
-//            (f90:1914-1918) of the previous iteration, then Ah = A h and the partials
-//            of h.Ah                                   (e3x_stencil.hip)
+//   stencil  (it >= 2) h = z + gamma h_prev or z (f90:1961-1996) and, every other
+//            iteration, the two pending u steps u -= lambda h (f90:1914-1918), then
+//            Ah = A h and the partials of h.Ah         (e3x_stencil.hip)
 //   fin_hah  hAh, |hAh| < 1e-14 break, lambda         (f90:1889-1907)
 //   update   g -= lambda Ah, z = M^-1 g, partials of z.g (f90:1919-1946)
 //   fin_upd  gg, rel_res, history, reset / gamma, stop test (f90:1949-2007)
-// The u step and the direction of the last iteration are applied once after the loop
-// (k_pcg_finalize).  The arithmetic of every value is the one of the separate steps.
+// The u steps still pending and the direction of the last iteration are applied once
+// after the loop (k_pcg_finalize).  The arithmetic of every value is the one of the separate steps.
 // Storage type T of g, h, z, Ah and M^-1 is double, or float in the mixed form; u and
 // every dot product stay double.
 #include "e3x_device.hpp"
@@ -35,8 +35,10 @@ template <typename T>
 struct PcgArgs {
     int64_t ns;
     int it, ldemb, kkk;
+    int usteps;     // DO_U: 1, or 2 with the step of iteration it-1 (from hm1) first
     double *u;
     T *g, *h, *z, *ah;
+    const T *hm1;   // h_{it-1}
     const T *minv;
     double *partials;
     PcgState *st;
@@ -117,6 +119,7 @@ __global__ __launch_bounds__(kFinBlock) void k_hah_finish(const double *partials
         S.exit_flag = 2;  // E3X_PCG_HAH_BREAK
         S.run_dir = 0;
     } else {
+        S.lambda_prev = S.lambda;
         S.lambda = S.gg / tot[0];
     }
 }
@@ -154,12 +157,13 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
                                                 const T *__restrict__ ah, const T *__restrict__ minv)
 {
     bool act[NR];
-    double lam[NR];
+    double lam[NR], lamp[NR];
     bool any = false;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         act[r] = A.st[r].stop_iter == 0;
         lam[r] = A.st[r].lambda;
+        lamp[r] = A.st[r].lambda_prev;
         any |= act[r];
     }
     if (!any) return;
@@ -180,7 +184,11 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
             const int64_t o = ((int64_t)r * ns + m) * 3;
             if constexpr (DO_U) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) u[o + c] = u[o + c] - lam[r] * (double)hv[o + c];
+                for (int c = 0; c < 3; c++) {
+                    double uc = u[o + c];
+                    if (A.usteps == 2) uc = uc - lamp[r] * (double)A.hm1[o + c];
+                    u[o + c] = uc - lam[r] * (double)hv[o + c];
+                }
             }
             double g[3];
 #pragma unroll
@@ -255,54 +263,63 @@ __global__ __launch_bounds__(kFinBlock) void k_update_finish(const double *parti
     }
 }
 
-// After the loop, per right-hand side that ended at iteration K by convergence or the
-// iteration limit: u -= lambda h_K unless iteration K already did (f90:1914-1918), and
-// the direction update that follows the stop test (f90:1961-1996, 2007) into h; where
-// the loop ended before that update (run_dir 0: the sweep and plain-CG loops on
-// convergence, A_2_0:1518, cg:1050) h_K itself.  After a curvature break (f90:1899-1902)
-// only h_K is moved into h.  h_K sits in h (K odd) or h_alt (K even).
+// After the loop, per right-hand side that ended at iteration K, the u steps still
+// pending (f90:1914-1918; the fused applies take them two at a time, u_done_after_apply):
+// after convergence or the iteration limit the steps up to K, unless iteration K replaced
+// the gradient (it then brought u up to K itself); after a curvature break (f90:1899-1902,
+// no step K) those up to K-1.  Then the direction update that follows the stop test
+// (f90:1961-1996, 2007) into h; where the loop ended before that update (run_dir 0: the
+// sweep and plain-CG loops on convergence, A_2_0:1518, cg:1050) or broke, h_K itself.
+// h_k sits in h (k odd) or h_alt (k even).
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_pcg_finalize(PcgArgs<T> A, T *h_alt, int replace_every)
 {
-    int mode[NR];   // 0 nothing, 1 step + direction, 2 copy h_K, 3 step + copy h_K
-    bool dou[NR], rst[NR];
-    double lam[NR], gam[NR];
+    int K[NR], first[NR], last[NR];  // pending u steps first..last (none when first > last)
+    int dir[NR];                     // 0 none, 1 direction update, 2 h_K into h
+    bool rst[NR];
+    double lamK[NR], lamK1[NR], gam[NR];  // lambdas of the steps K and K-1
     bool any = false;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const PcgState &S = A.st[r];
-        const int K = S.stop_iter;
-        mode[r] = 0;
-        if (K >= 1) {
-            if (S.exit_flag == 1 || S.exit_flag == 3) mode[r] = S.run_dir ? 1 : 3;
-            else if (!(K & 1)) mode[r] = 2;
+        const int k = S.stop_iter;
+        K[r] = k;
+        first[r] = 1;
+        last[r] = 0;
+        dir[r] = 0;
+        lamK[r] = lamK1[r] = 0.0;
+        if (k >= 1) {
+            const bool conv = S.exit_flag == 1 || S.exit_flag == 3;
+            first[r] = u_done_after_apply(k, replace_every) + 1;
+            last[r] = conv ? ((replace_every > 0 && k % replace_every == 0) ? 0 : k) : k - 1;
+            lamK[r] = conv ? S.lambda : 0.0;
+            lamK1[r] = conv ? S.lambda_prev : S.lambda;  // a break leaves lambda at step K-1
+            dir[r] = (conv && S.run_dir) ? 1 : 2;
         }
-        dou[r] = !(replace_every > 0 && K % replace_every == 0);
         rst[r] = S.reset != 0;
-        lam[r] = S.lambda;
         gam[r] = S.gamma;
-        any |= mode[r] != 0;
+        any |= first[r] <= last[r] || dir[r] != 0;
     }
     if (!any) return;
     const int64_t ns = A.ns;
     for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            if (!mode[r]) continue;
-            const T *hk = (A.st[r].stop_iter & 1) ? A.h : h_alt;
+            if (first[r] > last[r] && dir[r] == 0) continue;
+            const T *hk = (K[r] & 1) ? A.h : h_alt;    // h_K
+            const T *hk1 = (K[r] & 1) ? h_alt : A.h;   // h_{K-1}
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const int64_t o = ((int64_t)r * ns + m) * 3 + c;
                 const T hv = hk[o];
-                if (mode[r] == 2) {
-                    A.h[o] = hv;
-                } else if (mode[r] == 3) {
-                    if (dou[r]) A.u[o] = A.u[o] - lam[r] * (double)hv;
-                    if (hk != A.h) A.h[o] = hv;
-                } else {
-                    if (dou[r]) A.u[o] = A.u[o] - lam[r] * (double)hv;
-                    A.h[o] = rst[r] ? A.z[o] : (T)((double)A.z[o] + gam[r] * (double)hv);
+                if (first[r] <= last[r]) {
+                    double uc = A.u[o];
+                    if (first[r] <= K[r] - 1) uc = uc - lamK1[r] * (double)hk1[o];  // step K-1
+                    if (last[r] == K[r]) uc = uc - lamK[r] * (double)hv;            // step K
+                    A.u[o] = uc;
                 }
+                if (dir[r] == 1) A.h[o] = rst[r] ? A.z[o] : (T)((double)A.z[o] + gam[r] * (double)hv);
+                else if (dir[r] == 2 && hk != A.h) A.h[o] = hv;
             }
         }
     }
@@ -334,12 +351,13 @@ void init_nr(Solver &s, int kkk, int ldemb)
 }
 
 template <typename T, int NR>
-void iter_nr(Solver &s, int it, bool replace, bool prev_replace)
+void iter_nr(Solver &s, int it, int replace_every)
 {
     const int g = (int)vector_grid(s);
     const dim3 vg(g), blk(kBlock);
-    // 1. direction and u step of the previous iteration, Ah = A h, the curvature (f90:1578-1907)
-    (void)launch_apply_hah(s, sizeof(T) == 4, it, !prev_replace);
+    const bool replace = replace_every > 0 && it % replace_every == 0;
+    // 1. direction, the pending u steps, Ah = A h, the curvature (f90:1578-1907)
+    (void)launch_apply_hah(s, sizeof(T) == 4, it, it >= 2 ? u_steps_of_apply(it, replace_every) : 0);
     PcgArgs<T> A = pcg_args<T>(s, it);
     // 2. updates, z and z.g (f90:1914-2007)
     const bool gsq = s.stop_rule == 1;
@@ -347,10 +365,16 @@ void iter_nr(Solver &s, int it, bool replace, bool prev_replace)
         if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true, true>), vg, blk, 0, s.stream, A);
         else hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true>), vg, blk, 0, s.stream, A);
     } else {
-        // reliable update: u first (with this iteration's direction), then the true
-        // gradient g = K u + b in FP64, then z
-        if constexpr (sizeof(T) == 4) A.h = (it & 1) ? s.h32 : s.h32_alt;
-        else A.h = (it & 1) ? s.h : s.h_alt;
+        // reliable update: u first (the pending step of iteration it-1, if any, then this
+        // iteration's), then the true gradient g = K u + b in FP64, then z
+        if constexpr (sizeof(T) == 4) {
+            A.h = (it & 1) ? s.h32 : s.h32_alt;
+            A.hm1 = (it & 1) ? s.h32_alt : s.h32;
+        } else {
+            A.h = (it & 1) ? s.h : s.h_alt;
+            A.hm1 = (it & 1) ? s.h_alt : s.h;
+        }
+        A.usteps = it - u_done_after_apply(it, replace_every);
         hipLaunchKernelGGL((k_pcg_update<T, NR, false, true, false>), vg, blk, 0, s.stream, A);
         (void)launch_resid(s);
         if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true, true>), vg, blk, 0, s.stream, A);
@@ -385,15 +409,15 @@ void init_t(Solver &s, int kkk, int ldemb)
 }
 
 template <typename T>
-void iter_t(Solver &s, int it, bool replace, bool prev_replace)
+void iter_t(Solver &s, int it, int replace_every)
 {
     switch (s.nrhs) {
-    case 1: iter_nr<T, 1>(s, it, replace, prev_replace); break;
-    case 2: iter_nr<T, 2>(s, it, replace, prev_replace); break;
-    case 3: iter_nr<T, 3>(s, it, replace, prev_replace); break;
-    case 4: iter_nr<T, 4>(s, it, replace, prev_replace); break;
-    case 5: iter_nr<T, 5>(s, it, replace, prev_replace); break;
-    default: iter_nr<T, 6>(s, it, replace, prev_replace); break;
+    case 1: iter_nr<T, 1>(s, it, replace_every); break;
+    case 2: iter_nr<T, 2>(s, it, replace_every); break;
+    case 3: iter_nr<T, 3>(s, it, replace_every); break;
+    case 4: iter_nr<T, 4>(s, it, replace_every); break;
+    case 5: iter_nr<T, 5>(s, it, replace_every); break;
+    default: iter_nr<T, 6>(s, it, replace_every); break;
     }
 }
 
@@ -449,10 +473,10 @@ hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
     return hipGetLastError();
 }
 
-hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, bool replace, bool prev_replace)
+hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, int replace_every)
 {
-    if (mixed) iter_t<float>(s, it, replace, prev_replace);
-    else iter_t<double>(s, it, false, false);
+    if (mixed) iter_t<float>(s, it, replace_every);
+    else iter_t<double>(s, it, 0);
     return hipGetLastError();
 }
 

@@ -39,11 +39,12 @@ struct StencilArgs {
     double *partials;
     const PcgState *st;    // kApply, kApplyFused, kResid: skip finished right-hand sides
     // kApplyFused: x is formed as z + gamma hold (z on a reset) and written to hnew;
-    // with fuse_u, u -= lambda hold (gamma, reset, lambda of the previous iteration)
+    // with usteps = 2, u -= lambda_prev h_{it-2} (read from hnew before it is overwritten),
+    // then u -= lambda hold (gamma, reset, lambdas of the previous iterations)
     const T *z, *hold;
     T *hnew;
     double *u;
-    int fuse_u;
+    int usteps;
     int rmap[kMaxRhs];     // grid row j serves right-hand side rmap[j]
 };
 
@@ -131,11 +132,12 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // the solver state first, so that its latency overlaps the loads below (r < nrhs:
     // the grid has nrhs rows)
     int stop = 0, rst = 0;
-    double gam = 0.0, lam = 0.0;
+    double gam = 0.0, lam = 0.0, lamp = 0.0;
     if constexpr (HAS_ST) stop = A.st[r].stop_iter;
     if constexpr (FUSED) {
         gam = A.st[r].gamma;
         lam = A.st[r].lambda;
+        lamp = A.st[r].lambda_prev;
         rst = A.st[r].reset;
     }
     // Interior tiles index without wrapping; boundary tiles wrap each coordinate.
@@ -176,6 +178,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     T ai[3], as_[kShellPer][3];  // z (fused) or x
     T bi[3], bs_[kShellPer][3];  // h_prev (fused)
     double ui[3] = {0.0, 0.0, 0.0};
+    T hm2[3] = {T(0), T(0), T(0)};  // h_{it-2} at the own node (fused, usteps = 2)
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         ai[c] = src[gi + c];
@@ -187,9 +190,12 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
         }
     }
     if constexpr (FUSED) {
-        if (ingrid) {  // whether u is updated (fuse_u) is known later: a scalar wait here would hold these back
+        if (ingrid && A.usteps != 0) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) ui[c] = u[gi + c];
+            for (int c = 0; c < 3; c++) {
+                ui[c] = u[gi + c];
+                hm2[c] = hnew[gi + c];  // before this apply overwrites it with h_it
+            }
         }
     }
 
@@ -204,9 +210,9 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     for (int k = 0; k < kShellPer; k++) qs[k] = hpos_index(shp[k]);
     if constexpr (FUSED) {
         // this iteration's direction, h = z + gamma h_prev or h = z on a reset
-        // (f90:1961-1996); the tile's own nodes also store it and take the previous step
-        // into u, u -= lambda h_prev (f90:1914-1918)
-        const bool dou = A.fuse_u != 0;
+        // (f90:1961-1996); the tile's own nodes also store it and, every other iteration,
+        // take the two pending steps into u, u -= lambda h (f90:1914-1918), in order
+        const bool dou = A.usteps != 0;
         auto dir = [&](T zc, T hc) -> T { return rst != 0 ? zc : (T)((double)zc + gam * (double)hc); };
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -216,8 +222,11 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
             for (int k = 0; k < kShellPer; k++)
                 if (shv[k]) hs[c * HN + qs[k]] = dir(as_[k][c], bs_[k][c]);
             if (ingrid) {
+                if (dou) {
+                    const double u1 = ui[c] - lamp * (double)hm2[c];
+                    u[gi + c] = u1 - lam * (double)bi[c];
+                }
                 hnew[gi + c] = hv;
-                if (dou) u[gi + c] = ui[c] - lam * (double)bi[c];
             }
         }
     } else {
@@ -553,7 +562,7 @@ hipError_t launch_apply_plain(Solver &s, const double *x, double *y)
 // each right-hand side not yet known (on the host) to have stopped; the blocks of one
 // that stopped since still find that out from the device state.
 template <typename T>
-void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T *ah, int it, bool fuse_u,
+void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T *ah, int it, int usteps,
                 const int (&rmap)[kMaxRhs], int rows)
 {
     StencilArgs<T> A = stencil_args<T>(s, dk, sten, it <= 1 ? hbuf[1] : nullptr, ah);
@@ -562,12 +571,12 @@ void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T
         dispatch_stencil<T, kApply>(s, A, rows);
     } else {
         A.z = z; A.hold = hbuf[(it - 1) & 1]; A.hnew = hbuf[it & 1];
-        A.u = s.u; A.fuse_u = fuse_u ? 1 : 0;
+        A.u = s.u; A.usteps = usteps;
         dispatch_stencil<T, kApplyFused>(s, A, rows);
     }
 }
 
-hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u)
+hipError_t launch_apply_hah(Solver &s, bool mixed, int it, int usteps)
 {
     int rmap[kMaxRhs] = {}, rows = 0;
     for (int r = 0; r < s.nrhs; r++)
@@ -575,26 +584,27 @@ hipError_t launch_apply_hah(Solver &s, bool mixed, int it, bool fuse_u)
     if (rows == 0) return hipSuccess;  // every right-hand side has stopped: nothing to apply
     if (mixed) {
         float *hb[2] = {s.h32_alt, s.h32};
-        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, it, fuse_u, rmap, rows);
+        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, it, usteps, rmap, rows);
     } else {
         double *hb[2] = {s.h_alt, s.h};
-        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, it, fuse_u, rmap, rows);
+        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, it, usteps, rmap, rows);
     }
     return launch_hah_finish(s, (int)stencil_parts(s), it);
 }
 
-// The fused apply of an even iteration, as the PCG loop runs it (u and the state are
-// left to the caller: with lambda = gamma = 0 it leaves u unchanged and sets h = z).
-hipError_t launch_apply_only(Solver &s, bool mixed)
+// The fused apply of iteration 2 with usteps u steps, as the PCG loop runs it (u and the
+// state are left to the caller: with both lambdas and gamma 0 it leaves u unchanged and
+// sets h = z).
+hipError_t launch_apply_only(Solver &s, bool mixed, int usteps)
 {
     int all[kMaxRhs];
     for (int r = 0; r < kMaxRhs; r++) all[r] = r;
     if (mixed) {
         float *hb[2] = {s.h32_alt, s.h32};
-        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, 2, true, all, s.nrhs);
+        apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, 2, usteps, all, s.nrhs);
     } else {
         double *hb[2] = {s.h_alt, s.h};
-        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, 2, true, all, s.nrhs);
+        apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, 2, usteps, all, s.nrhs);
     }
     return hipGetLastError();
 }
