@@ -45,7 +45,7 @@ void release(Solver &s)
     (void)hipSetDevice(s.device);
     void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.perm, s.u, s.g, s.b, s.h, s.h_alt, s.z,
                    s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
-                   s.st, s.hist, s.results, s.post, s.hot};
+                   s.st, s.hist, s.results, s.post, s.hot, s.hphase, s.minv_hom, s.minv_hom32};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (s.results_host) (void)hipHostFree(s.results_host);
@@ -185,6 +185,9 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->z, nv));
     CK(dalloc(*s, &s->ah, nv));
     CK(dalloc(*s, &s->minv, 6 * ns));
+    CK(dalloc(*s, &s->hphase, ns));
+    CK(dalloc(*s, &s->minv_hom, (int64_t)nphase * 6));
+    CK(dalloc(*s, &s->minv_hom32, (int64_t)nphase * 6));
     CK(dalloc(*s, &s->cconst, kMaxRhs));
     // partials: [values][blocks] of the largest reduction (stress: 12 per rhs over the
     // streaming grid; energy: 2 per rhs over the stencil grid; rhs constant: 6)

@@ -91,6 +91,11 @@ struct Solver {
     // fields, [nrhs][ns][3] (the three components of a node side by side)
     double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;
     double *minv = nullptr;                                    // [ns][6]
+    // homogeneous phase of each node (its 8 voxels share it), 0xFFFF for the others, and
+    // the M^-1 block of a homogeneous node per phase ([nphase][6], FP64 and FP32)
+    uint16_t *hphase = nullptr;
+    double *minv_hom = nullptr;
+    float *minv_hom32 = nullptr;
     float *g32 = nullptr, *h32 = nullptr, *z32 = nullptr, *ah32 = nullptr, *minv32 = nullptr;
     // Second direction buffer: the direction of PCG iteration k lives in h (k odd) or
     // h_alt (k even), since the apply that forms it reads the previous one as halo.

@@ -40,6 +40,8 @@ struct PcgArgs {
     T *g, *h, *z, *ah;
     const T *hm1;   // h_{it-1}
     const T *minv;
+    const uint16_t *hphase;  // homogeneous phase of each node, 0xFFFF for the others
+    const T *minv_hom;       // [nphase][6]: M^-1 of a homogeneous node of each phase
     double *partials;
     PcgState *st;
     double *hist;
@@ -172,11 +174,18 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
 #pragma unroll
     for (int q = 0; q < (GSQ ? 2 * NR : NR); q++) part[q] = 0.0;
     for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
-        // the node's M^-1 block, once for all right-hand sides
+        // the node's M^-1 block, once for all right-hand sides: from the per-phase table
+        // for a homogeneous node (the same bits as its entry in minv), else from minv
         double mi[6];
         if constexpr (DO_Z) {
+            const int hp = A.hphase[m];
+            if (hp != 0xFFFF) {
 #pragma unroll
-            for (int q = 0; q < 6; q++) mi[q] = (double)load_once(&minv[6 * m + q]);
+                for (int q = 0; q < 6; q++) mi[q] = (double)A.minv_hom[6 * hp + q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; q++) mi[q] = (double)load_once(&minv[6 * m + q]);
+            }
         }
 #pragma unroll
         for (int r = 0; r < NR; r++) {
@@ -331,10 +340,11 @@ PcgArgs<T> pcg_args(const Solver &s, int it)
     PcgArgs<T> A{};
     A.ns = s.ns; A.it = it; A.u = s.u;
     A.partials = s.partials; A.st = s.st; A.hist = s.hist;
+    A.hphase = s.hphase;
     if constexpr (sizeof(T) == 4) {
-        A.g = s.g32; A.h = s.h32; A.z = s.z32; A.ah = s.ah32; A.minv = s.minv32;
+        A.g = s.g32; A.h = s.h32; A.z = s.z32; A.ah = s.ah32; A.minv = s.minv32; A.minv_hom = s.minv_hom32;
     } else {
-        A.g = s.g; A.h = s.h; A.z = s.z; A.ah = s.ah; A.minv = s.minv;
+        A.g = s.g; A.h = s.h; A.z = s.z; A.ah = s.ah; A.minv = s.minv; A.minv_hom = s.minv_hom;
     }
     return A;
 }

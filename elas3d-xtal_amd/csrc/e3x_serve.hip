@@ -76,8 +76,8 @@ int64_t footprint(int nx, int ny, int nz, int nphase_cap, bool mixed, int ldemb)
     const int64_t ns = (int64_t)nx * ny * nz;
     int64_t b = ns * 2;                                   // pix
     b += (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ) * kSB * (int64_t)sizeof(PermT);  // lane order
-    b += 7 * 3 * ns * 8 + 6 * ns * 8;                     // u g b h h_alt z Ah, M^-1
-    b += (int64_t)nphase_cap * ((36 + 576 + kStenHalf + 3) * 8 + (576 + kStenHalf) * 4);
+    b += 7 * 3 * ns * 8 + 6 * ns * 8 + ns * 2;            // u g b h h_alt z Ah, M^-1, homogeneous phase
+    b += (int64_t)nphase_cap * ((36 + 576 + kStenHalf + 3 + 6) * 8 + (576 + kStenHalf + 6) * 4);
     b += std::max<int64_t>(12 * 2048, 2 * ((ns + 255) / 256)) * 8 + (1 << 16);  // partials, state
     b += (int64_t)std::max(ldemb, 1) * 8;
     if (mixed) b += 5 * 3 * ns * 4 + 6 * ns * 4;

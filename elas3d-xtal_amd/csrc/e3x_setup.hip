@@ -316,27 +316,17 @@ __global__ __launch_bounds__(kBlock) void k_affine(int nx, int ny, int nz, int n
 }
 
 // ---------------------------------------------------------------------------
-// Block-Jacobi preconditioner (f90:1498-1520): M = sum over the 8 voxels of the node
-// of dk(p, a, :, a, :), inverted by cofactors (invert3x3 f90:2046-2147).  identity: the
-// plain-CG loop's M^-1 = I (z = 1 g0 + 0 g1 + 0 g2 = g exactly for finite g).
-__global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int64_t ns,
-                                                    const uint16_t *pix, const double *dk,
-                                                    double *minv, float *minv32, int identity)
+// Block-Jacobi preconditioner block of a node whose 8 voxels have phases ph (f90:1498-1520):
+// M = sum over the voxels a of dk(ph[a], a, :, a, :), inverted by cofactors (invert3x3
+// f90:2046-2147); o = ainv(1,1), (2,2), (3,3), (1,2), (1,3), (2,3).  One function for the
+// per-node array and the per-phase table of homogeneous nodes, without contraction, so
+// that both give the same bits.
+__device__ __forceinline__ void precond_block(const double *dk, const int (&ph)[8], double (&o)[6])
 {
-    const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (m >= ns) return;
-    if (identity) {
-        for (int q = 0; q < 6; q++) {
-            minv[6 * m + q] = q < 3 ? 1.0 : 0.0;
-            if (minv32) minv32[6 * m + q] = q < 3 ? 1.0f : 0.0f;
-        }
-        return;
-    }
-    const int i = (int)(m % nx), j = (int)((m / nx) % ny), k = (int)(m / ((int64_t)nx * ny));
+#pragma clang fp contract(off)
     double M[3][3];
     for (int a = 0; a < 8; a++) {
-        const int64_t v = wrap(i - PX(a), nx) + (int64_t)nx * (wrap(j - PY(a), ny) + (int64_t)ny * wrap(k - PZ(a), nz));
-        const double *D = dk + (int64_t)pix[v] * 576;
+        const double *D = dk + (int64_t)ph[a] * 576;
         for (int ii = 0; ii < 3; ii++)
             for (int jj = 0; jj < 3; jj++) {
                 const double d = D[((a * 3 + ii) * 8 + a) * 3 + jj];
@@ -347,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int6
     const double c21 = -M[1][0] * M[2][2] + M[1][2] * M[2][0];
     const double c31 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
     const double det = M[0][0] * c11 + M[0][1] * c21 + M[0][2] * c31;
-    double o[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < 6; q++) o[q] = 0.0;
     if (!(fabs(det) < 1e-14)) {
         const double inv = 1.0 / det;
         const double c12 = -M[0][1] * M[2][2] + M[0][2] * M[2][1];
@@ -359,9 +349,56 @@ __global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int6
         o[0] = c11 * inv; o[1] = c22 * inv; o[2] = c33 * inv;
         o[3] = c12 * inv; o[4] = c13 * inv; o[5] = c23 * inv;
     }
+}
+
+// Per node: M^-1 (and its FP32 copy), and hphase[m] = the phase of its 8 voxels when they
+// share one, else 0xFFFF; the streaming update takes homogeneous nodes' M^-1 from the
+// per-phase table (k_precond_table) and reads this array only for the others.  identity:
+// the plain-CG loop's M^-1 = I (z = 1 g0 + 0 g1 + 0 g2 = g exactly for finite g).
+__global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int64_t ns,
+                                                    const uint16_t *pix, const double *dk,
+                                                    double *minv, float *minv32, uint16_t *hphase, int identity)
+{
+    const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (m >= ns) return;
+    const int i = (int)(m % nx), j = (int)((m / nx) % ny), k = (int)(m / ((int64_t)nx * ny));
+    int ph[8];
+    bool homog = true;
+    for (int a = 0; a < 8; a++) {
+        const int64_t v = wrap(i - PX(a), nx) + (int64_t)nx * (wrap(j - PY(a), ny) + (int64_t)ny * wrap(k - PZ(a), nz));
+        ph[a] = pix[v];
+        homog &= ph[a] == ph[0];
+    }
+    hphase[m] = homog ? (uint16_t)ph[0] : (uint16_t)0xFFFF;
+    double o[6];
+    if (identity) {
+        for (int q = 0; q < 6; q++) o[q] = q < 3 ? 1.0 : 0.0;
+    } else {
+        precond_block(dk, ph, o);
+    }
     for (int q = 0; q < 6; q++) {
         minv[6 * m + q] = o[q];
         if (minv32) minv32[6 * m + q] = (float)o[q];
+    }
+}
+
+// M^-1 of a homogeneous node of each phase, [nphase][6] (the same function as k_precond).
+__global__ __launch_bounds__(64) void k_precond_table(int nphase, const double *dk, double *tab, float *tab32,
+                                                      int identity)
+{
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= nphase) return;
+    int ph[8];
+    for (int a = 0; a < 8; a++) ph[a] = p;
+    double o[6];
+    if (identity) {
+        for (int q = 0; q < 6; q++) o[q] = q < 3 ? 1.0 : 0.0;
+    } else {
+        precond_block(dk, ph, o);
+    }
+    for (int q = 0; q < 6; q++) {
+        tab[6 * p + q] = o[q];
+        if (tab32) tab32[6 * p + q] = (float)o[q];
     }
 }
 
@@ -405,9 +442,11 @@ hipError_t launch_affine(Solver &s)
 
 hipError_t launch_precond(Solver &s, bool want32)
 {
+    const int identity = s.pcg_variant == 2 ? 1 : 0;
     hipLaunchKernelGGL(k_precond, dim3((unsigned)vector_blocks(s)), dim3(kBlock), 0, s.stream, s.nx, s.ny,
-                       s.nz, s.ns, s.pix, s.dk, s.minv, want32 ? s.minv32 : nullptr,
-                       s.pcg_variant == 2 ? 1 : 0);
+                       s.nz, s.ns, s.pix, s.dk, s.minv, want32 ? s.minv32 : nullptr, s.hphase, identity);
+    hipLaunchKernelGGL(k_precond_table, dim3((unsigned)((s.nphase + 63) / 64)), dim3(64), 0, s.stream, s.nphase,
+                       (const double *)s.dk, s.minv_hom, want32 ? s.minv_hom32 : nullptr, identity);
     s.precond_identity = s.pcg_variant == 2;
     return hipGetLastError();
 }
