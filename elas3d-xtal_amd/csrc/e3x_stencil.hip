@@ -454,7 +454,7 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
     // nodes out by their LDS bank residue (halo index mod 32), round by round, so that the
     // 16- and 32-lane groups that read LDS together hit distinct banks where the run allows
     // it.  Key: run start, rank of the node among the run's nodes of its residue, residue.
-    static_assert(kSB <= 256, "the run key holds 8-bit positions");
+    static_assert(kSB <= 4096, "the run key holds 12-bit positions");
     const unsigned long long mine = key[t];
     int start = t;
     while (start > 0 && (key[start - 1] >> 12) == (mine >> 12)) start--;
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
     int rank = 0;
     for (int i = start; i < t; i++) rank += residue((int)(key[i] & 0xFFFull)) == res;
     __syncthreads();
-    key[t] = ((unsigned long long)start << 24) | ((unsigned long long)rank << 16) | ((unsigned long long)res << 8) |
+    key[t] = ((unsigned long long)start << 40) | ((unsigned long long)rank << 24) | ((unsigned long long)res << 12) |
              (unsigned long long)node;
     __syncthreads();
     for (int k = 2; k <= kSB; k <<= 1)
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
             }
             __syncthreads();
         }
-    perm[(int64_t)bid * kSB + t] = (PermT)(key[t] & 0xFFull);
+    perm[(int64_t)bid * kSB + t] = (PermT)(key[t] & 0xFFFull);
 }
 
 template <int NR>
