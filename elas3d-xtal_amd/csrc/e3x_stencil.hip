@@ -25,9 +25,30 @@
 namespace e3x {
 namespace {
 
+// n / d for 0 <= n < 2^31 by a multiply and a shift (d fixed per launch): with
+// 2^s <= d < 2^(s+1) and M = ceil(2^(32+s) / d) <= 2^32, the error M d - 2^(32+s) < d
+// keeps floor(n M / 2^(32+s)) exact.  Uniform operands stay in scalar registers (the
+// hardware has no integer divide; a division is otherwise a vector reciprocal sequence).
+struct FastDiv {
+    unsigned long long m = 1;
+    int sh = 32;
+};
+inline FastDiv fast_div(unsigned d)
+{
+    int s = 0;
+    while ((2ull << s) <= d) s++;
+    FastDiv f;
+    f.m = ((1ull << (32 + s)) + d - 1) / d;
+    f.sh = 32 + s;
+    return f;
+}
+__device__ __forceinline__ int fdiv(int n, FastDiv f) { return (int)(((unsigned long long)(unsigned)n * f.m) >> f.sh); }
+
 template <typename T>
 struct StencilArgs {
     int nx, ny, nz, ntx, nty, nrhs;
+    FastDiv d_ntile, d_ntx, d_nty;  // by the tiles of one right-hand side, tiles along x, along y
+    int wrap1;                      // every extent >= 8: a halo index wraps at most once
     int64_t ns;
     const uint16_t *pix;
     const PermT *perm;     // lane order per tile (Solver::perm)
@@ -122,8 +143,10 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     // find the most recent z lines still in the last-level cache (-4 % at 200^3 FP64).
     const int ntile = gridDim.x, nrhs = A.nrhs;
     const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
-    const int r = A.rmap[lin / ntile], bid = ntile - 1 - lin % ntile;
-    const int tx = bid % A.ntx, ty = (bid / A.ntx) % A.nty, tz = bid / (A.ntx * A.nty);
+    const int row = fdiv(lin, A.d_ntile), bid = ntile - 1 - (lin - row * ntile);
+    const int r = A.rmap[row];
+    const int txy = fdiv(bid, A.d_ntx), tz = fdiv(txy, A.d_nty);
+    const int tx = bid - txy * A.ntx, ty = txy - tz * A.nty;
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     const int64_t ns = A.ns, o = (int64_t)(3 * r) * ns;
@@ -145,7 +168,8 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     const int corner = (oz - 1) * nxy + (oy - 1) * nx + (ox - 1);
     auto gpos = [&](HPos p) -> int {
         return interior ? corner + p.z * nxy + p.y * nx + p.x
-                        : pmod(ox + p.x - 1, nx) + nx * pmod(oy + p.y - 1, ny) + nxy * pmod(oz + p.z - 1, nz);
+                        : A.wrap1 ? wrap(ox + p.x - 1, nx) + nx * wrap(oy + p.y - 1, ny) + nxy * wrap(oz + p.z - 1, nz)
+                                  : pmod(ox + p.x - 1, nx) + nx * pmod(oy + p.y - 1, ny) + nxy * pmod(oz + p.z - 1, nz);
     };
 
     // --- loads -------------------------------------------------------------------
@@ -504,6 +528,10 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
     StencilArgs<T> A{};
     A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.nrhs = s.nrhs;
     A.ntx = (s.nx + TX - 1) / TX; A.nty = (s.ny + TY - 1) / TY;
+    A.d_ntile = fast_div((unsigned)stencil_blocks(s));
+    A.d_ntx = fast_div((unsigned)A.ntx);
+    A.d_nty = fast_div((unsigned)A.nty);
+    A.wrap1 = s.nx >= 8 && s.ny >= 8 && s.nz >= 8;
     A.ns = s.ns; A.pix = s.pix; A.perm = s.perm; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
     A.partials = s.partials; A.st = s.st;
     for (int r = 0; r < kMaxRhs; r++) A.rmap[r] = r;
