@@ -479,17 +479,29 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
     // 16- and 32-lane groups that read LDS together hit distinct banks where the run allows
     // it.  Key: run start, rank of the node among the run's nodes of its residue, residue.
     static_assert(kSB <= 4096, "the run key holds 12-bit positions");
+    __shared__ unsigned char resv[kSB];  // bank residue of the node at each sorted position
     const unsigned long long mine = key[t];
-    int start = t;
-    while (start > 0 && (key[start - 1] >> 12) == (mine >> 12)) start--;
-    auto residue = [](int node) {
+    const int node = (int)(mine & 0xFFFull);
+    int res;
+    {
         const int w = node >> 6, l = node & 63;
         const int x = wave_x(w) + (l & 3), y = wave_y(w) + ((l >> 2) & 3), z = wave_z(w) + (l >> 4);
-        return ((z + 1) * (HX * HY) + (y + 1) * HX + (x + 1)) & 31;
-    };
-    const int node = (int)(mine & 0xFFFull), res = residue(node);
+        res = ((z + 1) * (HX * HY) + (y + 1) * HX + (x + 1)) & 31;
+    }
+    resv[t] = (unsigned char)res;
+    // start of this lane's run: inclusive prefix maximum of the run heads
+    __shared__ int sst[kSB];
+    sst[t] = (t == 0 || (key[t - 1] >> 12) != (mine >> 12)) ? t : 0;
+    __syncthreads();
+    for (int off = 1; off < kSB; off <<= 1) {
+        const int v = t >= off ? max(sst[t], sst[t - off]) : sst[t];
+        __syncthreads();
+        sst[t] = v;
+        __syncthreads();
+    }
+    const int start = sst[t];
     int rank = 0;
-    for (int i = start; i < t; i++) rank += residue((int)(key[i] & 0xFFFull)) == res;
+    for (int i = start; i < t; i++) rank += resv[i] == res;
     __syncthreads();
     key[t] = ((unsigned long long)start << 40) | ((unsigned long long)rank << 24) | ((unsigned long long)res << 12) |
              (unsigned long long)node;

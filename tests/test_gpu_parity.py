@@ -407,15 +407,16 @@ def test_pcg_variants_match_oracle(variant, precision):
         assert _rel(u, r_ref["u"]) < 1e-10
 
 
-@pytest.mark.parametrize("ldemb,replace_every", [(15, 0), (16, 0), (17, 0), (17, 8)])
+@pytest.mark.parametrize("ldemb,replace_every", [(15, 0), (16, 0), (17, 0), (17, 8), (18, 7), (16, 3), (14, 1),
+                                                  (15, 2)])
 def test_mixed_steps_reach_u_once_in_order(ldemb, replace_every):
     """The mixed form stopped by the iteration limit after an odd or an even number of
-    iterations (its u step of iteration k lands in the apply of iteration k+1, or in
-    the update of a replacement iteration, or after the loop), without and with
-    replacements every 8 iterations: the PCG correction u - u0 matches the FP64
-    oracle's after the same number of iterations to FP32 rounding; one step lost or
-    applied twice is off by more than ten times the tolerance (consecutive iterates
-    here differ by ~1e-4 of it)."""
+    iterations, without and with replacements every 1, 2, 3, 7 or 8 iterations.  The u
+    steps land two at a time in the applies, or in the update of a replacement
+    iteration, or after the loop.  The PCG correction u - u0 must match the FP64
+    oracle's after the same number of iterations, to FP32 rounding.  One step lost or
+    applied twice is off by more than ten times the tolerance: consecutive iterates here
+    differ by ~1e-4 of it."""
     shape = (12, 12, 12)
     pix, rod = voronoi_case(shape, 5, seed=400)
     e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
@@ -455,16 +456,19 @@ def test_mixed_precision_against_fp64_oracle(replace_every):
     assert _rel(res.strain6[0], ro["strain6"]) < 1e-12
 
 
-@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
-def test_multi_rhs_equals_single_solves(precision):
+@pytest.mark.parametrize("precision,replace_every", [(L.PREC_FP64, 0), (L.PREC_MIXED, 50), (L.PREC_MIXED, 7),
+                                                    (L.PREC_MIXED, 6)])
+def test_multi_rhs_equals_single_solves(precision, replace_every):
     """Three right-hand sides in one PCG give each one's single solve (they stop at
-    different iterations)."""
+    different iterations), also with true-gradient replacements every 7 or 6 iterations
+    (the u steps pending at a replacement: two after an even count of iterations since
+    the last one, one after an odd count)."""
     shape = (12, 10, 8)
     pix, rod = voronoi_case(shape, 6, seed=500)
     strains = np.array([[oc.AML_F32, 0, 0, 0, 0, 0], [0, 0, 0, 0.5e-3, 0, 0], [0, 0.7e-3, 0, 0, 0, 0.2e-3]])
-    batch = solve(pix, rod, shape, 7, C11, C12, C44, strains, precision=precision)
+    batch = solve(pix, rod, shape, 7, C11, C12, C44, strains, precision=precision, replace_every=replace_every)
     for r in range(3):
-        one = solve(pix, rod, shape, 7, C11, C12, C44, strains[r], precision=precision)
+        one = solve(pix, rod, shape, 7, C11, C12, C44, strains[r], precision=precision, replace_every=replace_every)
         assert batch.lstep[r] == one.lstep[0]
         assert abs(batch.utot[r] - one.utot[0]) <= 1e-13 * abs(one.utot[0])
         assert _rel(batch.stress6[r], one.stress6[0]) < 1e-13
