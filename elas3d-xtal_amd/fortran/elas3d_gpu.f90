@@ -10,6 +10,9 @@
 !   output_SS316L_polycrystal.txt, cg_history_polycrystal.txt,
 !   cgitr_SS316L_polycrystal.txt                               logs  (f90:278, 466, 1458)
 !   fullfield_poly.h5           (/pix, /vm, /stress_tensor)     write (f90:2840)
+!   top_peaks.txt               the MATLAB post-processing's hotspots
+!                               (applications/sec3_2/A_2_0/elas3dxtal_postprocessing.m
+!                               :86-243), when hot_threshold_mpa > 0
 ! The grid extents come from the /pix 'dimensions' attribute when present.
 !------------------------------------------------------------------------------
       program elas3d_gpu
@@ -31,16 +34,23 @@
       real :: tol = 1.0d-8                          ! `real` (f90:148)
       integer :: precision = E3X_PREC_FP64, replace_every = 50, device = 0
       integer :: stop_rule = E3X_STOP_REFERENCE, pcg_variant = E3X_PCG_REFERENCE
+      ! hotspots (top_peaks.txt): off unless hot_threshold_mpa > 0; sigma, peaks and the
+      ! voxel coordinates x = hot_x0 + i * hot_dx (i 0-based) as the script's xvec
+      real(dp) :: hot_threshold_mpa = -1.0d0, hot_sigma = 0.4d0, hot_dx = 1.0d0, hot_x0 = 0.0d0
+      integer :: hot_npeaks = 3
       character(len=512) :: input_file = 'input_structure_poly.h5'
       character(len=512) :: output_dir = '.'
       namelist /elas3d/ nx, ny, nz, n_grains, c11_local, c12_local, c44_local, e1, nu1, &
                         k_matrix, g_matrix, flag_m, aml, aml_exx, aml_eyy, aml_ezz, aml_exz, &
                         aml_eyz, aml_exy, kmax, ldemb, tol, precision, replace_every, device, &
-                        stop_rule, pcg_variant, input_file, output_dir
+                        stop_rule, pcg_variant, hot_threshold_mpa, hot_sigma, hot_npeaks, hot_dx, &
+                        hot_x0, input_file, output_dir
 
       ! ---- state ------------------------------------------------------------------
       type(c_ptr) :: h = c_null_ptr
-      integer :: nphase, i, kkk, ltot, lhist, ios, nargs
+      integer :: nphase, i, kkk, ltot, lhist, ios, nargs, npk
+      integer(c_int64_t) :: hidx(64), nfound, m
+      real(dp) :: hval(64)
       integer(int64) :: ns
       integer(c_int64_t) :: npix, nori
       integer(c_int32_t) :: dims(3)
@@ -210,6 +220,47 @@
       call e3x_check(e3x_stress_fullfield(h, 0, stress_field, vm), 'stress_fullfield')
       call e3x_check(e3x_h5_write_fullfield(odir//'fullfield_poly.h5'//c_null_char, ns, pix, vm, &
                                             stress_field), 'fullfield HDF5 write', io=.true.)
+
+      ! ---- hotspots of the MATLAB post-processing (postprocessing.m:86-243) ---------
+      ! solid = every label up to max(pix), as the script sets it (.m:67); the first
+      ! min(3, found) peaks in the script's format (.m:232-243)
+      if (hot_threshold_mpa > 0.0d0) then
+        npk = max(1, min(hot_npeaks, 64))
+        call e3x_check(e3x_hotspots(h, 0, c_null_ptr, int(maxval(pix), c_int), hot_sigma, &
+                                    hot_threshold_mpa, int(npk, c_int), hidx, hval, nfound), 'hotspots')
+        if (nfound == 0) then
+          print *, 'No hotspots found exceeding the threshold!'
+        else
+          npk = int(min(3_c_int64_t, min(int(npk, c_int64_t), nfound)))
+          open(unit=101, file=odir//'top_peaks.txt', status='replace')
+          write(101, '(A,I0,A)') 'Top ', npk, ' Local Maxima (Von Mises Stress):'
+          do i = 1, npk
+            m = hidx(i)
+            write(101, '(A,I0,A,A,A,A,A,A,A,A,A)') 'Peak #', i, ': ', &
+                fmt_f(real(real(hval(i), c_float) / 1.0e6_c_float, dp), 3), ' MPa at (', &
+                fmt_f(hot_x0 + real(mod(m, int(nx, c_int64_t)), dp) * hot_dx, 2), ', ', &
+                fmt_f(hot_x0 + real(mod(m / nx, int(ny, c_int64_t)), dp) * hot_dx, 2), ', ', &
+                fmt_f(hot_x0 + real(m / (int(nx, c_int64_t) * ny), dp) * hot_dx, 2), ')'
+          end do
+          close(101)
+        end if
+      end if
       call e3x_check(e3x_destroy(h), 'e3x_destroy')
       close(7)
+      contains
+
+      ! x with d decimals and no padding, as C's %.<d>f prints it
+      function fmt_f(x, d) result(s)
+        real(dp), intent(in) :: x
+        integer, intent(in) :: d
+        character(len=:), allocatable :: s
+        character(len=48) :: buf, f
+        write(f, '(A,I0,A)') '(F48.', d, ')'
+        write(buf, f) x
+        s = trim(adjustl(buf))
+        if (s(1:1) == '.') s = '0'//s
+        if (len(s) > 1) then
+          if (s(1:2) == '-.') s = '-0'//s(2:)
+        end if
+      end function fmt_f
       end program elas3d_gpu

@@ -20,20 +20,36 @@ def test_fortran_driver_matches_oracle(tmp_path):
     shape, ngrain = (18, 16, 14), 9
     pix, rod, _, _ = io.generate(shape, ngrain, 77, pore_mode=io.PORES_VOXELS, pore_fraction=0.02)
     io.write_input(str(tmp_path / "input_structure_poly.h5"), shape, pix, rod)
-    (tmp_path / "elas3d.nml").write_text(
-        "&elas3d\n c11_local = 246.5d9, c12_local = 147.3d9, c44_local = 124.7d9,\n"
-        " aml_exx = 1.0000000474974513d-3, aml_ezz = 0.0d0\n/\n")
-    r = subprocess.run([DRIVER, "elas3d.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
-    print(r.stdout[-2000:], r.stderr[-2000:])
-    assert r.returncode == 0
     e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
     ro = oc.solve(shape, ngrain + 1, pix, rod, C11, C12, C44, e6)
-    ns = int(np.prod(shape))
-    p2, vm, st = io.read_fullfield(str(tmp_path / "fullfield_poly.h5"), ns)
-    assert np.array_equal(p2, pix)
     rot = oc.rotate_stiffness(rod, C11, C12, C44)
     cm = oc.cmod(ngrain + 1, rot, [0.0, 0.0])
     sf_o, vm_o = oc.fullfield(shape, pix, cm, e6, ro["u"])
+    thr = float(np.percentile(vm_o / 1e6, 97))   # the hotspot threshold of top_peaks.txt, MPa
+    (tmp_path / "elas3d.nml").write_text(
+        "&elas3d\n c11_local = 246.5d9, c12_local = 147.3d9, c44_local = 124.7d9,\n"
+        " aml_exx = 1.0000000474974513d-3, aml_ezz = 0.0d0,\n"
+        f" hot_threshold_mpa = {thr!r}, hot_dx = 2.5d0, hot_x0 = -20.0d0\n/\n")
+    r = subprocess.run([DRIVER, "elas3d.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    ns = int(np.prod(shape))
+    p2, vm, st = io.read_fullfield(str(tmp_path / "fullfield_poly.h5"), ns)
+    assert np.array_equal(p2, pix)
+    # top_peaks.txt (postprocessing.m:232-243): the oracle's hotspots of its own field,
+    # positions exact, values to the printed precision
+    idx_o, val_o, n_o = oc.hotspots(shape, pix, vm_o, ngrain + 1, sigma=0.4, thr_mpa=thr, npeaks=3)
+    tp = (tmp_path / "top_peaks.txt").read_text().splitlines()
+    k = min(3, n_o)
+    assert tp[0] == f"Top {k} Local Maxima (Von Mises Stress):" and len(tp) == k + 1
+    nx, ny = shape[0], shape[1]
+    for line, m, v in zip(tp[1:], idx_o[:k], val_o[:k]):
+        head, _, pos = line.partition(" MPa at (")
+        assert head.startswith("Peak #")
+        assert abs(float(head.split(": ")[1]) - float(np.float32(v) / np.float32(1e6))) < 2e-3
+        got = [float(c) for c in pos.rstrip(")").split(", ")]
+        want = [-20.0 + 2.5 * c for c in (m % nx, (m // nx) % ny, m // (nx * ny))]
+        assert got == want, (line, want)
     assert np.max(np.abs(st - sf_o)) < 1e-9 * np.max(np.abs(sf_o))
     assert np.max(np.abs(vm - vm_o)) < 1e-9 * np.max(np.abs(vm_o))
     lines = (tmp_path / "cgitr_SS316L_polycrystal.txt").read_text().split("\n")
