@@ -355,6 +355,28 @@ def test_isotropic_matrix_void_inclusion_matches_oracle(precision):
     assert 0 < res.stress6[0][2] < (k0 + 4 * g0 / 3) * oc.AML_F32
 
 
+@pytest.mark.parametrize("variant,stop_rule,precision", [
+    (L.PCG_SWEEP, L.STOP_REFERENCE, L.PREC_FP64), (L.PCG_PLAIN_CG, L.STOP_REFERENCE, L.PREC_MIXED),
+    (L.PCG_PLAIN_CG, L.STOP_NIST_ABS, L.PREC_FP64), (L.PCG_REFERENCE, L.STOP_NIST_ABS, L.PREC_MIXED)])
+def test_loop_options_per_rhs_equal_single_solves(variant, stop_rule, precision):
+    """The loop variants and the NIST stop rule with three right-hand sides in one PCG:
+    each stops where its single solve does, with the same energy.  Moduli in GPa for the
+    NIST rule (its units); in Pa for the reference's (in GPa its absolute hAh floor,
+    f90:1899, would end the loop first)."""
+    unit = 1e-9 if stop_rule == L.STOP_NIST_ABS else 1.0
+    c11, c12, c44 = C11 * unit, C12 * unit, C44 * unit
+    shape = (12, 10, 9)
+    pix, rod = voronoi_case(shape, 5, seed=430)
+    e3 = np.array([[oc.AML_F32, 0, 0, 0, 0, 0], [0, 0, 0, 0.8e-3, 0, 0], [0, 0.4e-3, 0, 0, 0, -0.3e-3]])
+    kw = dict(precision=precision, replace_every=7, stop_rule=stop_rule, variant=variant)
+    many = solve(pix, rod, shape, 6, c11, c12, c44, e3, **kw)
+    for r in range(3):
+        one = solve(pix, rod, shape, 6, c11, c12, c44, e3[r], **kw)
+        assert many.exit[r] == one.exit[0] == L.PCG_CONVERGED
+        assert many.lstep[r] == one.lstep[0]
+        assert abs(many.utot[r] - one.utot[0]) <= 1e-13 * abs(one.utot[0])
+
+
 @pytest.mark.parametrize("variant", [L.PCG_SWEEP, L.PCG_PLAIN_CG])
 @pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
 def test_pcg_variants_match_oracle(variant, precision):
