@@ -5,19 +5,21 @@
 // 9x9x5 voxel phases in LDS.  A node is homogeneous when its 8 voxels share one phase,
 // mixed otherwise.  The lanes take the tile's nodes in an order fixed when the
 // microstructure is loaded (k_tile_order): homogeneous nodes first, by phase, then the
-// mixed ones by their phases, so that most waves hold one kind and few phases.  Coefficients are wave-uniform
-// and read through the scalar cache (SGPR operands, no per-lane coefficient loads):
+// mixed ones by their phases, so that most waves hold one kind and few phases.
+// Coefficients are wave-uniform and read through the scalar cache (SGPR operands, no
+// per-lane coefficient loads):
 //   - homogeneous lanes: the wave loops over the phases its lanes hold; for phase P the
 //     27-point stencil of P (243 FMA; the block of -delta is the transpose of the block
 //     of +delta, so 126 coefficients) goes to the lanes of phase P;
-//   - mixed lanes: for each octant a in order 0..7 the wave reads the 24 neighbour
-//     values of voxel a once, then loops over the phases its lanes have at octant a and
-//     applies the 3x24 rows of the element matrix dk[P][a] (72 FMA) to those lanes.
+//   - mixed lanes: for each octant, in the pairs (a, a^6) of opposite corners, the wave
+//     reads the 24 neighbour values of that voxel once, then loops over the phases its
+//     lanes have there and applies 3x24 rows of the element matrix (72 FMA) to those
+//     lanes; both octants of a pair use the rows dk[P][a], the second in permuted order.
 // The next phase is the one of the first lane still pending (ballot + readlane).  Each
 // lane is served exactly once per octant (or once in all, when homogeneous), so a
 // node's sum runs over its octants in fixed order whatever its neighbours hold: results
 // are bitwise reproducible.  Each block serves one right-hand side, so the
-// kernel is the same for any number of them (17 KB of LDS in FP64).
+// kernel is the same for any number of them (22 KB of LDS in FP64).
 #include "e3x_device.hpp"
 
 #include <cmath>
