@@ -325,6 +325,32 @@ def test_pcg_variants_match_dense_restatement(variant):
     assert _rel(r["h"], d["h"]) < 1e-4       # h is of the size of the final residual
 
 
+def test_pcg_directions_are_conjugate():
+    """Preconditioned CG with the Fletcher-Reeves gamma: successive directions are
+    K-conjugate (h_i' K h_j = 0, i != j) and the preconditioned gradients M^-1-orthogonal
+    (g_i' M^-1 g_j = 0).  A wrong gamma, a transposed block or a dropped term breaks it."""
+    shape = (4, 4, 3)
+    ns = int(np.prod(shape))
+    pix, dk, u0, g0, gg0 = _pcg_case(shape, 3, seed=55)
+    kg = ind.assemble_dense(shape, pix, ind.dk_to_kmats(dk))
+    minv = ind.block_jacobi_dense(kg, ns)
+    runs = [oc.dembx(shape, pix, dk, u0, g0, gg0, tol=0.0, ldemb=k) for k in range(1, 6)]
+    hs = [r["h"] for r in runs]                        # directions 2..6
+    for i in range(len(hs)):
+        for j in range(i):
+            c = abs(hs[i] @ kg @ hs[j]) / np.sqrt((hs[i] @ kg @ hs[i]) * (hs[j] @ kg @ hs[j]))
+            assert c < 1e-8, (i, j, c)
+    gs = [g0] + [r["gb"] for r in runs]
+
+    def prec(v):
+        return np.einsum("mij,mj->mi", minv, v.reshape(3, ns).T).T.ravel()
+
+    for i in range(len(gs)):
+        for j in range(i):
+            c = abs(gs[i] @ prec(gs[j])) / np.sqrt((gs[i] @ prec(gs[i])) * (gs[j] @ prec(gs[j])))
+            assert c < 1e-8, (i, j, c)
+
+
 def test_plain_cg_residuals_are_orthogonal():
     """Textbook CG (M = I, Fletcher-Reeves gamma): successive gradients are mutually
     orthogonal.  A preconditioner left in, or a wrong gamma, breaks it."""
