@@ -49,7 +49,7 @@ void release(Solver &s)
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (s.results_host) (void)hipHostFree(s.results_host);
-    if (s.flags_host) (void)hipHostFree(s.flags_host);
+    if (s.chk_host) (void)hipHostFree(s.chk_host);
     if (s.st_host) (void)hipHostFree(s.st_host);
     for (hipEvent_t &e : s.ev)
         if (e) (void)hipEventDestroy(e);
@@ -197,7 +197,7 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->st, kMaxRhs));
     CK(dalloc(*s, &s->results, 12 * kMaxRhs));
     CK(hipHostMalloc((void **)&s->results_host, sizeof(double) * 12 * kMaxRhs, hipHostMallocDefault));
-    CK(hipHostMalloc((void **)&s->flags_host, sizeof(int) * 2 * kMaxRhs, hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&s->chk_host, sizeof(PcgState) * 2 * kMaxRhs, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&s->st_host, sizeof(PcgState) * 3 * kMaxRhs, hipHostMallocDefault));
     CK(hipMemset(s->h, 0, sizeof(double) * (size_t)nv));
     CK(hipMemset(s->u, 0, sizeof(double) * (size_t)nv));
@@ -393,8 +393,43 @@ int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const doubl
     return E3X_OK;
 }
 
+// The iteration at which every running right-hand side is expected to have stopped, from
+// the slope of its log residual (the stop rule's quantity) between its last two snapshots;
+// 0 when one of them cannot be predicted (no earlier snapshot, a residual that did not
+// fall, as after a divergence reset, or a stop test that cannot be met).  It only sizes
+// the chunks: the kernels take the stop decision on the device.
+static int predict_stop(Solver &s, const PcgState *snap, int last)
+{
+    Solver::PcgRun &R = s.run;
+    int pred = 0;
+    bool all = true;
+    for (int r = 0; r < s.nrhs; r++) {
+        const PcgState &q = snap[r];
+        if (q.stop_iter != 0) continue;
+        const bool nist = q.stop_rule == 1;
+        const double v = nist ? q.gsq : q.rel_res, target = nist ? q.gsq_tol : q.tol;
+        int p = 0;
+        if (v > 0.0 && target > 0.0 && std::isfinite(v)) {
+            const double lv = std::log(v);
+            if (R.seen_it[r] > 0 && last > R.seen_it[r]) {
+                const double rate = (lv - R.seen_lv[r]) / (last - R.seen_it[r]);
+                if (rate < 0.0) {
+                    const double need = std::min(std::ceil((std::log(target) - lv) / rate), 1e6);
+                    p = last + std::max(1, (int)need);
+                }
+            }
+            R.seen_it[r] = last;
+            R.seen_lv[r] = lv;
+        }
+        if (p == 0) all = false;
+        pred = std::max(pred, p);
+    }
+    return all ? pred : 0;
+}
+
 // Every kernel of the loop reads the per-rhs stop state on the device, so the host only
-// polls the stop flags (double-buffered, one chunk behind) to stop launching.
+// polls the state (double-buffered, one chunk behind) to stop launching.  A chunk ends one
+// iteration past the predicted stop; past that, chunks are a quarter of the usual length.
 int e3x_dembx_advance(e3x_solver *h, int block, int *done)
 {
     if (int rc = check_handle(h)) return rc;
@@ -404,20 +439,23 @@ int e3x_dembx_advance(e3x_solver *h, int block, int *done)
     if (!done) return fail(E3X_EINVAL, "done is NULL");
     *done = 0;
     E3X_HIP(hipSetDevice(s.device));
-    int *flags[2] = {s.flags_host, s.flags_host + kMaxRhs};
     while (!R.finishing) {
         // keep up to two chunks ahead of the last one examined
         while (!R.stop_seen && R.it <= R.ldemb && R.k - R.checked < 2) {
-            const int n = std::min(R.chunk, R.ldemb - R.it + 1);
+            int n = std::min(R.chunk, R.ldemb - R.it + 1);
+            if (R.pred > 0) {
+                const int left = R.pred + 1 - R.it + 1;  // through one iteration past the prediction
+                n = std::min(n, left > 0 ? left : std::max(1, R.chunk / 4));
+            }
             for (int t = 0; t < n; t++) {
                 const int itt = R.it + t;
                 E3X_HIP(launch_pcg_iter(s, R.mixed, itt, R.mixed ? R.replace_every : 0));
             }
             R.it += n;
-            // the stop_iter of every rhs into the pinned slot of this chunk
-            for (int r = 0; r < s.nrhs; r++)
-                E3X_HIP(hipMemcpyAsync(flags[R.k & 1] + r, &s.st[r].stop_iter, sizeof(int), hipMemcpyDeviceToHost,
-                                       s.stream));
+            // the state of every rhs into the pinned slot of this chunk
+            R.chunk_last[R.k & 1] = R.it - 1;
+            E3X_HIP(hipMemcpyAsync(s.chk_host + (R.k & 1) * kMaxRhs, s.st, sizeof(PcgState) * s.nrhs,
+                                   hipMemcpyDeviceToHost, s.stream));
             E3X_HIP(hipEventRecord(s.ev[2 + (R.k & 1)], s.stream));
             R.k++;
         }
@@ -430,12 +468,14 @@ int e3x_dembx_advance(e3x_solver *h, int block, int *done)
                 if (q == hipErrorNotReady) return E3X_OK;
                 E3X_HIP(q);
             }
+            const PcgState *snap = s.chk_host + (R.checked & 1) * kMaxRhs;
             bool all = true;
             for (int r = 0; r < s.nrhs; r++) {
-                const bool stopped = flags[R.checked & 1][r] != 0;
+                const bool stopped = snap[r].stop_iter != 0;
                 all &= stopped;
                 if (stopped) s.pcg_known_stopped |= 1u << r;
             }
+            R.pred = predict_stop(s, snap, R.chunk_last[R.checked & 1]);
             R.checked++;
             if (all) R.stop_seen = true;
             continue;

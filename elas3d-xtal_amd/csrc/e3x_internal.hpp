@@ -115,15 +115,21 @@ struct Solver {
     int64_t hist_cap = 0;
     double *results = nullptr;     // small device result slots
     double *results_host = nullptr;  // pinned mirror
-    int *flags_host = nullptr;       // pinned stop flags
+    PcgState *chk_host = nullptr;    // pinned PCG state snapshots, two chunks' worth
     std::vector<int64_t> counts;     // phase histogram of pix (assig)
 
     // PCG in progress (e3x_dembx_begin / _advance / _end): chunks of iterations are
-    // enqueued ahead; the stop flags of chunk k land in flags_host[k & 1], marked by
-    // ev[2 + (k & 1)]; at most two chunks are unexamined at a time.
+    // enqueued ahead; the state of every right-hand side after chunk k lands in
+    // chk_host[(k & 1) * kMaxRhs + r], marked by ev[2 + (k & 1)]; at most two chunks are
+    // unexamined at a time.  From the residuals of the chunks examined the host predicts
+    // the stop iteration and ends the chunks there, so that few launches run past it.
     struct PcgRun {
         bool on = false, mixed = false, stop_seen = false, finishing = false;
         int replace_every = 0, ldemb = 0, chunk = 0, it = 1, k = 0, checked = 0;
+        int chunk_last[2] = {0, 0};  // the last iteration of the chunk in slot k & 1
+        int pred = 0;                // predicted stop iteration (0: none)
+        int seen_it[kMaxRhs] = {};   // per rhs: iteration and log residual of the last snapshot
+        double seen_lv[kMaxRhs] = {};
     } run;
     // right-hand sides the host has seen stop in the PCG in progress (bit r): the apply
     // launches no blocks for them
