@@ -385,6 +385,7 @@ int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const doubl
     Solver::PcgRun &R = s.run;
     R = Solver::PcgRun{};
     s.pcg_known_stopped = 0;
+    s.pcg_check_first = false;
     R.on = true;
     R.mixed = mixed;
     R.replace_every = mixed ? replace_every : 0;
@@ -449,8 +450,10 @@ int e3x_dembx_advance(e3x_solver *h, int block, int *done)
             }
             for (int t = 0; t < n; t++) {
                 const int itt = R.it + t;
+                s.pcg_check_first = R.pred > 0 && itt > R.pred;
                 E3X_HIP(launch_pcg_iter(s, R.mixed, itt, R.mixed ? R.replace_every : 0));
             }
+            s.pcg_check_first = false;
             R.it += n;
             // the state of every rhs into the pinned slot of this chunk
             R.chunk_last[R.k & 1] = R.it - 1;

@@ -134,6 +134,9 @@ struct Solver {
     // right-hand sides the host has seen stop in the PCG in progress (bit r): the apply
     // launches no blocks for them
     uint32_t pcg_known_stopped = 0;
+    // the iteration being enqueued lies past the predicted stop: its apply reads the stop
+    // state before it loads its tile
+    bool pcg_check_first = false;
     int stop_rule = 0;  // e3x_stop_rule of the next PCG
     int pcg_variant = 0;  // e3x_pcg_variant of the next PCG
     bool precond_identity = false;  // minv (minv32) hold the identity (E3X_PCG_PLAIN_CG)

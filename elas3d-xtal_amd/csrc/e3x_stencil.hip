@@ -68,6 +68,7 @@ struct StencilArgs {
     T *hnew;
     double *u;
     int usteps;
+    int check_first;       // read the stop state before loading (launches past the predicted stop)
     int rmap[kMaxRhs];     // grid row j serves right-hand side rmap[j]
 };
 
@@ -153,6 +154,10 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     const int64_t ns = A.ns, o = (int64_t)(3 * r) * ns;
     const int tid = threadIdx.x;
+    if constexpr (HAS_ST) {
+        // a launch the host expects to find its right-hand side stopped checks before it loads
+        if (A.check_first != 0 && A.st[r].stop_iter != 0) return;  // uniform over the block
+    }
     if (tid == 0) arrived = 0u;  // ordered before use by the barrier below
     // the solver state first, so that its latency overlaps the loads below (r < nrhs:
     // the grid has nrhs rows)
@@ -614,6 +619,7 @@ void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T
     } else {
         A.z = z; A.hold = hbuf[(it - 1) & 1]; A.hnew = hbuf[it & 1];
         A.u = s.u; A.usteps = usteps;
+        A.check_first = s.pcg_check_first ? 1 : 0;
         dispatch_stencil<T, kApplyFused>(s, A, rows);
     }
 }
