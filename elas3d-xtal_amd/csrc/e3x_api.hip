@@ -450,7 +450,7 @@ int e3x_dembx_advance(e3x_solver *h, int block, int *done)
             }
             for (int t = 0; t < n; t++) {
                 const int itt = R.it + t;
-                s.pcg_check_first = R.pred > 0 && itt > R.pred;
+                s.pcg_check_first = R.pred > 0 && itt >= R.pred - 1;  // the prediction is good to ~2
                 E3X_HIP(launch_pcg_iter(s, R.mixed, itt, R.mixed ? R.replace_every : 0));
             }
             s.pcg_check_first = false;
