@@ -360,10 +360,32 @@ void init_nr(Solver &s, int kkk, int ldemb)
                        s.st, ldemb);
 }
 
+// The streaming update's grid: at most one round of resident blocks.  With six
+// right-hand sides the body takes 82 VGPRs (five waves per SIMD), and the 2048-block
+// grid-stride grid would leave its last blocks a second, partly empty round.
+template <typename T, int NR>
+int update_grid(const Solver &s)
+{
+    static int cap = 0;  // resident blocks on the device, per kernel instance
+    if (cap == 0) {
+        int per_cu = 0, cu = 0, dev = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_update<T, NR, true, false, true>, kBlock, 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            per_cu = kVecGridMax;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+            cu = 1;
+        cap = per_cu * cu;
+    }
+    const int64_t g = vector_grid(s);
+    return (int)(g < cap ? g : cap);
+}
+
 template <typename T, int NR>
 void iter_nr(Solver &s, int it, int replace_every)
 {
-    const int g = (int)vector_grid(s);
+    const int g = update_grid<T, NR>(s);
     const dim3 vg(g), blk(kBlock);
     const bool replace = replace_every > 0 && it % replace_every == 0;
     // 1. direction, the pending u steps, Ah = A h, the curvature (f90:1578-1907)
