@@ -415,8 +415,9 @@ static int predict_stop(Solver &s, const PcgState *snap, int last)
             if (R.seen_it[r] > 0 && last > R.seen_it[r]) {
                 const double rate = (lv - R.seen_lv[r]) / (last - R.seen_it[r]);
                 if (rate < 0.0) {
-                    const double need = std::min(std::ceil((std::log(target) - lv) / rate), 1e6);
-                    p = last + std::max(1, (int)need);
+                    // iterations still to go, clamped before the conversion to int
+                    const double need = std::min(std::max(std::ceil((std::log(target) - lv) / rate), 1.0), 1e6);
+                    p = last + (int)need;
                 }
             }
             R.seen_it[r] = last;
