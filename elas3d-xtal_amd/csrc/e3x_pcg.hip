@@ -15,6 +15,7 @@
 // every dot product stay double.
 #include "e3x_device.hpp"
 
+#include <atomic>
 #include <cmath>
 
 namespace e3x {
@@ -366,7 +367,10 @@ void init_nr(Solver &s, int kkk, int ldemb)
 template <typename T, int NR>
 int update_grid(const Solver &s)
 {
-    static int cap = 0;  // resident blocks on the device, per kernel instance
+    // resident blocks on the device, per kernel instance; handles on several host threads
+    // may race to fill it in, all with the same value
+    static std::atomic<int> cached{0};
+    int cap = cached.load(std::memory_order_relaxed);
     if (cap == 0) {
         int per_cu = 0, cu = 0, dev = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_update<T, NR, true, false, true>, kBlock, 0) !=
@@ -377,6 +381,7 @@ int update_grid(const Solver &s)
             hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
             cu = 1;
         cap = per_cu * cu;
+        cached.store(cap, std::memory_order_relaxed);
     }
     const int64_t g = vector_grid(s);
     return (int)(g < cap ? g : cap);
