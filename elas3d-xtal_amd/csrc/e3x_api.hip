@@ -83,6 +83,12 @@ int ensure_mixed_buffers(Solver &s)
     E3X_HIP(dalloc(s, &s.z32, nv));
     E3X_HIP(dalloc(s, &s.ah32, nv));
     E3X_HIP(dalloc(s, &s.minv32, 6 * s.ns));
+    // the buffers a fused apply reads before any PCG has written them (z and the
+    // direction of iteration 1, e.g. under e3x_time_apply) start as zeros, not as stale
+    // bits that lambda = 0 would turn into NaN
+    E3X_HIP(hipMemsetAsync(s.z32, 0, sizeof(float) * (size_t)nv, s.stream));
+    E3X_HIP(hipMemsetAsync(s.h32, 0, sizeof(float) * (size_t)nv, s.stream));
+    E3X_HIP(hipMemsetAsync(s.h32_alt, 0, sizeof(float) * (size_t)nv, s.stream));
     return E3X_OK;
 }
 
@@ -199,7 +205,11 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(hipHostMalloc((void **)&s->results_host, sizeof(double) * 12 * kMaxRhs, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&s->chk_host, sizeof(PcgState) * 2 * kMaxRhs, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&s->st_host, sizeof(PcgState) * 3 * kMaxRhs, hipHostMallocDefault));
+    // fields read before anything writes them (h and z by a fused apply timed before any
+    // PCG, e3x_time_apply) start as zeros
     CK(hipMemset(s->h, 0, sizeof(double) * (size_t)nv));
+    CK(hipMemset(s->h_alt, 0, sizeof(double) * (size_t)nv));
+    CK(hipMemset(s->z, 0, sizeof(double) * (size_t)nv));
     CK(hipMemset(s->u, 0, sizeof(double) * (size_t)nv));
 #undef CK
     *out = reinterpret_cast<e3x_solver *>(s);
@@ -515,7 +525,13 @@ int e3x_dembx_end(e3x_solver *h, int *lstep, double *gg, double *history, int *e
     Solver::PcgRun &R = s.run;
     if (!R.on) return fail(E3X_ESTATE, "e3x_dembx_end without e3x_dembx_begin");
     int done = 0;
-    if (int rc = e3x_dembx_advance(h, 1, &done)) return rc;
+    if (int rc = e3x_dembx_advance(h, 1, &done)) {
+        // the solve is abandoned: the handle takes calls again
+        R = Solver::PcgRun{};
+        s.pcg_known_stopped = 0;
+        s.pcg_check_first = false;
+        return rc;
+    }
     R.on = false;
     const int ldemb = R.ldemb;
     float ms0 = 0, ms1 = 0;
@@ -547,7 +563,7 @@ int e3x_dembx_ex(e3x_solver *h, int precision, int replace_every, const double *
     if (int rc = e3x_dembx_begin(h, precision, replace_every, gg0, tol, ldemb, kkk)) return rc;
     int done = 0;
     if (int rc = e3x_dembx_advance(h, 1, &done)) {
-        S(h).run.on = false;
+        S(h).run = Solver::PcgRun{};
         return rc;
     }
     return e3x_dembx_end(h, lstep, gg, history, exit_flag, reset_count);

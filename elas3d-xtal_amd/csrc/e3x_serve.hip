@@ -68,6 +68,13 @@ struct Server {
 Server &SV(e3x_server *p) { return *reinterpret_cast<Server *>(p); }
 const Server &SV(const e3x_server *p) { return *reinterpret_cast<const Server *>(p); }
 
+// The mixed form's FP32 buffers (ensure_mixed_buffers): g h h_alt z Ah and M^-1.
+int64_t mixed_bytes(int nx, int ny, int nz)
+{
+    const int64_t ns = (int64_t)nx * ny * nz;
+    return 5 * 3 * ns * 4 + 6 * ns * 4;
+}
+
 // Device bytes of a one-rhs handle created for nphase_cap phases, with the mixed form's
 // buffers and an iteration history of ldemb (the allocations of e3x_create,
 // ensure_mixed_buffers and ensure_hist).
@@ -78,9 +85,11 @@ int64_t footprint(int nx, int ny, int nz, int nphase_cap, bool mixed, int ldemb)
     b += (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ) * kSB * (int64_t)sizeof(PermT);  // lane order
     b += 7 * 3 * ns * 8 + 6 * ns * 8 + ns * 2;            // u g b h h_alt z Ah, M^-1, homogeneous phase
     b += (int64_t)nphase_cap * ((36 + 576 + kStenHalf + 3 + 6) * 8 + (576 + kStenHalf + 6) * 4);
-    b += std::max<int64_t>(12 * 2048, 2 * ((ns + 255) / 256)) * 8 + (1 << 16);  // partials, state
+    // partials (the larger of the streaming and the stencil reductions), state
+    const int64_t tiles = (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ);
+    b += std::max<int64_t>(12 * (int64_t)kVecGridMax, 2 * tiles) * 8 + (1 << 16);
     b += (int64_t)std::max(ldemb, 1) * 8;
-    if (mixed) b += 5 * 3 * ns * 4 + 6 * ns * 4;
+    if (mixed) b += mixed_bytes(nx, ny, nz);
     return b;
 }
 
@@ -91,30 +100,60 @@ int64_t handle_bytes(e3x_solver *h)
     return b;
 }
 
+// A handle's booking follows what it holds: its mixed buffers are allocated on its first
+// mixed solve (the admission booked them beforehand, retire_handle settles the difference).
 void retire_handle(Server &S, Handle hd)
 {
-    const int64_t now_bytes = handle_bytes(hd.h);  // grows by the mixed buffers on first use
+    const int64_t now_bytes = handle_bytes(hd.h);
     S.used += now_bytes - hd.bytes;
     hd.bytes = now_bytes;
     S.pool.push_back(hd);
 }
 
-// An idle handle of this grid size and enough phases, or one made room for and created.
-int acquire_handle(Server &S, const e3x_request &rq, int64_t need, Handle *out, bool *fits)
+// A handle whose solve failed part way is not reused: its state is unknown.
+void drop_handle(Server &S, Handle hd)
 {
-    *fits = true;
-    for (auto it = S.pool.begin(); it != S.pool.end(); ++it)
-        if (it->nx == rq.nx && it->ny == rq.ny && it->nz == rq.nz && it->nphase_cap >= rq.nphase) {
-            *out = *it;
-            S.pool.erase(it);
-            return E3X_OK;
-        }
-    // free idle handles of other sizes, least recently used first, until it fits
-    while (S.used + need > S.budget && !S.pool.empty()) {
+    S.used -= hd.bytes;
+    (void)e3x_destroy(hd.h);
+}
+
+bool has_mixed_buffers(e3x_solver *h) { return reinterpret_cast<Solver *>(h)->g32 != nullptr; }
+
+// Free idle handles, least recently used first, until `extra` more bytes fit.
+void make_room(Server &S, int64_t extra)
+{
+    while (S.used + extra > S.budget && !S.pool.empty()) {
         S.used -= S.pool.front().bytes;
         (void)e3x_destroy(S.pool.front().h);
         S.pool.pop_front();
     }
+}
+
+// An idle handle of this grid size and enough phases, or one made room for and created.
+// Every path books the bytes the request will hold before it is admitted: a pooled
+// handle that lacks the mixed buffers a mixed request allocates is booked for them too.
+int acquire_handle(Server &S, const e3x_request &rq, int64_t need, Handle *out, bool *fits)
+{
+    *fits = true;
+    const bool mixed = rq.precision == E3X_PREC_MIXED;
+    for (auto it = S.pool.begin(); it != S.pool.end(); ++it)
+        if (it->nx == rq.nx && it->ny == rq.ny && it->nz == rq.nz && it->nphase_cap >= rq.nphase) {
+            Handle hd = *it;
+            const int64_t extra = (mixed && !has_mixed_buffers(hd.h)) ? mixed_bytes(rq.nx, rq.ny, rq.nz) : 0;
+            S.pool.erase(it);
+            make_room(S, extra);
+            if (S.used + extra > S.budget) {
+                S.pool.push_front(hd);  // back among the idle handles; the request waits
+                *fits = false;
+                return E3X_OK;
+            }
+            S.used += extra;
+            hd.bytes += extra;
+            *out = hd;
+            return E3X_OK;
+        }
+    // free idle handles of other sizes, least recently used first, until it fits
+    make_room(S, need);
     if (S.used + need > S.budget) {
         *fits = false;
         return E3X_OK;
@@ -123,7 +162,9 @@ int acquire_handle(Server &S, const e3x_request &rq, int64_t need, Handle *out, 
     hd.nx = rq.nx; hd.ny = rq.ny; hd.nz = rq.nz;
     hd.nphase_cap = std::max(kPoolPhases, rq.nphase);
     if (int rc = e3x_create(rq.nx, rq.ny, rq.nz, hd.nphase_cap, 1, S.device, &hd.h)) return rc;
-    hd.bytes = handle_bytes(hd.h);
+    // booked at the admission estimate (which includes the mixed buffers and the history a
+    // mixed request allocates later); retire_handle settles it to what the handle holds
+    hd.bytes = std::max(handle_bytes(hd.h), need);
     S.used += hd.bytes;
     *out = hd;
     return E3X_OK;
@@ -186,7 +227,8 @@ int progress(Server &S, bool *changed)
         o.t_admit = a.t_admit;
         o.t_done = S.now();
         S.done.push_back(o);
-        retire_handle(S, a.hd);
+        if (o.status == E3X_OK) retire_handle(S, a.hd);
+        else drop_handle(S, a.hd);
         S.active.erase(S.active.begin() + (std::ptrdiff_t)i);
         *changed = true;
     }
@@ -222,7 +264,9 @@ int progress(Server &S, bool *changed)
             S.done.push_back(o);
             if (reinterpret_cast<Solver *>(a.hd.h)->run.on)
                 (void)e3x_dembx_end(a.hd.h, nullptr, nullptr, nullptr, nullptr, nullptr);
-            retire_handle(S, a.hd);
+            // an input error (a bad label, say) leaves the handle sound; a device error may not
+            if (rc == E3X_EPHASE || rc == E3X_EINVAL) retire_handle(S, a.hd);
+            else drop_handle(S, a.hd);
             continue;
         }
         S.active.push_back(std::move(a));

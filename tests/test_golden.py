@@ -50,21 +50,28 @@ def test_gpu_matches_golden(name):
     from elas3dxtal.solver import solve
     g = _load(name)
     shape = tuple(int(v) for v in g["shape"])
-    res, s = solve(g["pix"], g["rod"], shape, int(g["nphase"]), C11, C12, C44, g["strain6"],
-                   phasemod=g["phasemod"], flag_m=int(g["flag_m"]), keep=True)
+    args = (g["pix"], g["rod"], shape, int(g["nphase"]), C11, C12, C44, g["strain6"])
+    kw = dict(phasemod=g["phasemod"], flag_m=int(g["flag_m"]), keep=True)
+    res, s = solve(*args, **kw)
+    reset = name.startswith("reset")
+    lstep = int(g["lstep"])
+    K = int(res.lstep[0])
+    print(f"{name}: lstep GPU {K}, golden {lstep}")
     try:
+        assert res.exit[0] == int(g["status"]) == L.PCG_CONVERGED
+        assert res.reset_count[0] == int(g["reset_count"])
+        assert abs(K - lstep) <= max(1, int(0.01 * lstep))
+        if K != lstep:
+            # equal iteration counts: the GPU run for exactly the golden's lstep iterations
+            # (tol = 0, the iteration limit ends it) holds the iterate the file froze
+            s.close()
+            res, s = solve(*args, tol=0.0, ldemb=lstep, **kw)
+            assert res.exit[0] == L.PCG_MAXITER and int(res.lstep[0]) == lstep
         u = s.get(L.FIELD_U)
         sf, vm = s.stress_fullfield(0) if "vm" in g else (None, None)
     finally:
         s.close()
-    reset = name.startswith("reset")
-    assert res.exit[0] == int(g["status"]) == L.PCG_CONVERGED
-    assert res.reset_count[0] == int(g["reset_count"])
-    lstep = int(g["lstep"])
-    assert abs(int(res.lstep[0]) - lstep) <= max(1, int(0.01 * lstep))
     assert abs(res.utot0[0] - float(g["utot0"])) < 1e-12 * abs(float(g["utot0"]))
-    if res.lstep[0] != lstep:
-        return
     tol = 1e-6 if reset else 1e-10
     assert abs(res.utot[0] - float(g["utot"])) < (1e-8 if reset else 1e-10) * abs(float(g["utot"]))
     assert _rel(u, g["u"]) < tol

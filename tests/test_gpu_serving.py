@@ -69,7 +69,8 @@ def _handle_bytes(shape, mixed, ldemb=10000, phases=1024):
     b = ns * 2 + 7 * 3 * ns * 8 + 6 * ns * 8 + ns * 2
     b += -(-shape[0] // 8) * -(-shape[1] // 8) * -(-shape[2] // 4) * 256   # stencil lane order
     b += phases * ((36 + 576 + 126 + 3 + 6) * 8 + (576 + 126 + 6) * 4)
-    b += max(12 * 2048, 2 * ((ns + 255) // 256)) * 8 + (1 << 16) + ldemb * 8
+    tiles = -(-shape[0] // 8) * -(-shape[1] // 8) * -(-shape[2] // 4)
+    b += max(12 * 2048, 2 * tiles) * 8 + (1 << 16) + ldemb * 8
     if mixed:
         b += 5 * 3 * ns * 4 + 6 * ns * 4
     return b
