@@ -72,28 +72,63 @@ def solve_resident(s, c11, c12, c44, e6, precision):
     _, gg0 = s.energy()
     d = s.dembx(gg0, precision=precision, replace_every=50, want_history=False)
     ut, _ = s.energy()
-    st, _ = s.stress()
-    return time.perf_counter() - t0, d, ut, st
+    st, sn = s.stress()
+    return time.perf_counter() - t0, d, ut, st, sn
 
 
 def run_gpu(w, c11, c12, c44, e6, steps, warmup, precision):
+    """`steps` timed solves on the resident microstructure.  The stencil applies of the
+    timed PCG loops are bracketed by HIP events on the solver's stream (their mean is the
+    roofline's launch time); after the timed steps the outputs of the last solve are
+    copied back for the error table."""
     s = Elas3D(w["shape"], w["nphase"])
     try:
         s.ppixel(w["pix"], w["rod"])
         for _ in range(warmup):
             solve_resident(s, c11, c12, c44, e6, precision)
-        walls, pcg_ms, lsteps, ut = [], [], [], None
+        s.set_apply_timing(True)
+        walls, pcg_ms, lsteps, ut, apply_ms, apply_n = [], [], [], None, [], 0
         for _ in range(steps):
-            wall, d, ut, st = solve_resident(s, c11, c12, c44, e6, precision)
+            wall, d, ut, st, sn = solve_resident(s, c11, c12, c44, e6, precision)
             walls.append(wall)
             pcg_ms.append(d["pcg_ms"])
             lsteps.append(int(d["lstep"][0]))
             exit_flag = int(d["exit"][0])
-        apply_ms = s.time_apply(precision=precision, reps=20)
-        return dict(walls=walls, pcg_ms=pcg_ms, lsteps=lsteps, exit=exit_flag, utot=float(ut[0]),
-                    stress_xx=float(st[0][0]), apply_ms=apply_ms)
+            n, ms = s.last_apply_timing()
+            apply_ms.append(ms * n)
+            apply_n += n
+        s.set_apply_timing(False)
+        replay_ms = s.time_apply(precision=precision, reps=20)
+        _, vm = s.stress_fullfield()
+        u = s.get(L.FIELD_U)
+        s.affine_guess()
+        u_aff = s.get(L.FIELD_U)
+        return dict(u_affine=u_aff, walls=walls, pcg_ms=pcg_ms, lsteps=lsteps, exit=exit_flag, utot=float(ut[0]),
+                    stress6=st[0].copy(), strain6=sn[0].copy(), u=u, vm=vm,
+                    apply_ms=sum(apply_ms) / max(apply_n, 1), apply_launches=apply_n, replay_ms=replay_ms)
     finally:
         s.close()
+
+
+def output_errors(mx, ref):
+    """Error of every output of the mixed solve against the FP64 solve of the same
+    workload: energy, each averaged stress and strain (relative to the largest component
+    of its tensor), and the largest pointwise error of the displacement and von Mises
+    fields (relative to the field's largest magnitude)."""
+    names = ["xx", "yy", "zz", "xz", "yz", "xy"]
+    smax, emax = np.max(np.abs(ref["stress6"])), np.max(np.abs(ref["strain6"]))
+    out = {"utot": abs(mx["utot"] - ref["utot"]) / abs(ref["utot"])}
+    for k, n in enumerate(names):
+        out["stress_" + n] = float(abs(mx["stress6"][k] - ref["stress6"][k]) / smax)
+    for k, n in enumerate(names):
+        out["strain_" + n] = float(abs(mx["strain6"][k] - ref["strain6"][k]) / emax)
+    out["u_field_max"] = float(np.max(np.abs(mx["u"] - ref["u"])) / np.max(np.abs(ref["u"])))
+    # the displacement less its affine part E x (f90:423-438), which both forms hold exactly
+    corr = ref["u"] - ref["u_affine"]
+    out["u_correction_max"] = float(np.max(np.abs(mx["u"] - ref["u"])) / np.max(np.abs(corr)))
+    out["u_field_rms"] = float(np.sqrt(np.mean((mx["u"] - ref["u"]) ** 2)) / np.sqrt(np.mean(ref["u"] ** 2)))
+    out["vm_field_max"] = float(np.max(np.abs(mx["vm"] - ref["vm"])) / np.max(np.abs(ref["vm"])))
+    return out
 
 
 def run_six(w, c11, c12, c44, steps, warmup, precision):
@@ -118,37 +153,42 @@ def run_six(w, c11, c12, c44, steps, warmup, precision):
         s.close()
 
 
-def run_serving(n_requests, interarrival_ms, max_active, seed=5000):
-    """Serving stream: requests with Poisson arrivals, grid edge uniform over
-    {64, 96, 128, 160, 200}, 10-800 grains, Zener ratio A uniform on [0.5, 5], porosity
-    uniform on [0, 5] % as lognormal spheres, one random unit macro strain (1e-3) each,
-    mixed precision with replacement every 50 iterations.  The microstructures come
-    from a pool of two per grid edge, generated before the timed stream (host
-    generation of one per request would dominate it); every request draws its own pool
-    entry, A and load case.  One host thread submits at the arrival times and drives the
-    server between them."""
+def serving_requests(n_requests, seed=5000):
+    """The serving stream's requests, each with its own seeded microstructure: grid edge
+    uniform over {64, 96, 128, 160, 200}, 10-800 Voronoi grains with uniform SO(3)
+    orientations, Zener ratio A uniform on [0.5, 5], porosity uniform on [0, 5] % as
+    lognormal spheres (median radius 3 voxels, sigma 0.4), one random unit macro strain
+    of magnitude 1e-3; mixed precision, replacement every 50 iterations.  Generated on the
+    host before the stream starts."""
     from elas3dxtal import io
-    from elas3dxtal.serve import Server
 
     rng = np.random.default_rng(seed)
     edges = [64, 96, 128, 160, 200]
-    pool = []
-    for e in edges:
-        for rep in range(2):
-            ng = int(rng.integers(10, 801))
-            frac = float(rng.uniform(0.0, 0.05))
-            pix, rod, _, _ = io.generate((e, e, e), ng, 1000 * 5 + len(pool), pore_mode=io.PORES_SPHERES,
-                                         pore_fraction=frac, pore_median=3.0, pore_sigma=0.4)
-            pool.append(dict(shape=(e, e, e), nphase=ng + 1, pix=pix, rod=rod))
     reqs = []
-    for _ in range(n_requests):
-        m = pool[int(rng.integers(0, len(pool)))]
+    for k in range(n_requests):
+        e = int(edges[int(rng.integers(0, len(edges)))])
+        ng = int(rng.integers(10, 801))
+        frac = float(rng.uniform(0.0, 0.05))
+        pix, rod, _, _ = io.generate((e, e, e), ng, 1000 * 5 + k, pore_mode=io.PORES_SPHERES,
+                                     pore_fraction=frac, pore_median=3.0, pore_sigma=0.4)
         a = float(rng.uniform(0.5, 5.0))
         e6 = np.zeros(6)
         j = int(rng.integers(0, 6))
         e6[j] = 1e-3 if j < 3 else 0.5e-3
-        reqs.append((m, (configs.C12_GPA + 2.0 * configs.C44_GPA / a) * 1e9, e6))
-    gaps = rng.exponential(interarrival_ms * 1e-3, n_requests)
+        reqs.append(dict(shape=(e, e, e), nphase=ng + 1, pix=pix, rod=rod,
+                         c11=(configs.C12_GPA + 2.0 * configs.C44_GPA / a) * 1e9, e6=e6))
+    return reqs
+
+
+def run_serving(reqs, interarrival_ms, max_active, seed=5001):
+    """Serving stream over `reqs` with Poisson arrivals (mean gap interarrival_ms; 0: all
+    at once).  One host thread submits at the arrival times and drives the server between
+    them; latency is arrival to result."""
+    from elas3dxtal.serve import Server
+
+    n_requests = len(reqs)
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(interarrival_ms * 1e-3, n_requests) if interarrival_ms > 0 else np.zeros(n_requests)
     arrivals = np.cumsum(gaps) - gaps[0]
     with Server(max_active=max_active) as sv:
         t0 = time.perf_counter()
@@ -157,9 +197,9 @@ def run_serving(n_requests, interarrival_ms, max_active, seed=5000):
         while len(done) < n_requests:
             now = time.perf_counter() - t0
             while k < n_requests and arrivals[k] <= now:
-                m, c11, e6 = reqs[k]
-                sv.submit(m["pix"], m["rod"], m["shape"], m["nphase"], c11, configs.C12_GPA * 1e9, configs.C44_GPA * 1e9, e6,
-                          precision=L.PREC_MIXED, replace_every=50)
+                q = reqs[k]
+                sv.submit(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], configs.C12_GPA * 1e9,
+                          configs.C44_GPA * 1e9, q["e6"], precision=L.PREC_MIXED, replace_every=50)
                 k += 1
             sv.progress()
             done += sv.take()
@@ -177,13 +217,14 @@ def run_serving(n_requests, interarrival_ms, max_active, seed=5000):
         "latency_ms_p99": float(np.percentile(lat, 99) * 1e3),
         "dof_updates_per_s": float(dofups / wall),
         "wall_s": wall,
-        "pool": "2 microstructures per grid edge {64,96,128,160,200}, 10-800 grains, 0-5 % pores",
+        "microstructures": "one per request, seeded, generated before the stream",
     }
 
 
 def cpu_baseline(w, c11, c12, c44, e6, iters):
     """The oracle (C/OpenMP restatement of the reference) on this host: `iters` PCG
-    iterations on the same 200^3 workload, timed around oc_dembx only."""
+    iterations on the same 200^3 workload; timed is the iteration loop alone (the oracle's
+    own clock around f90:1574-2012's loop), as the GPU's t_pcg is."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as oc
 
@@ -195,14 +236,13 @@ def cpu_baseline(w, c11, c12, c44, e6, iters):
     u0 = oc.affine(shape, e6)
     g0, _ = oc.energy(shape, pix, dk, b, cval, u0)
     gg0 = oc.sumsq(g0)
-    t0 = time.perf_counter()
     r = oc.dembx(shape, pix, dk, u0, g0, gg0, ldemb=iters)
-    dt = time.perf_counter() - t0
+    dt = oc.lib.oc_last_loop_seconds()
     ns = int(np.prod(shape))
     return dict(value=3.0 * ns * r["lstep"] / dt, unit="DOF-updates/s", cores=int(oc.lib.oc_get_threads()),
                 kind="port",
-                sample=f"{r['lstep']} PCG iterations (block-Jacobi setup included) of the 200^3 A=2 workload "
-                       f"in {dt:.1f} s: oracle C restatement of dembx_OpenMP, OpenMP")
+                sample=f"the PCG iteration loop alone (block-Jacobi setup excluded), {r['lstep']} iterations of "
+                       f"the 200^3 A=2 workload in {dt:.1f} s: oracle C restatement of dembx_OpenMP, OpenMP")
 
 
 def profile_run(n, zener):
@@ -246,7 +286,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-six", action="store_true", help="skip the six-load-case (effective stiffness) runs")
     ap.add_argument("--six-steps", type=int, default=2)
-    ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=256, help="serving stream length (0: skip)")
     ap.add_argument("--serve-interarrival-ms", type=float, default=70.0)
     ap.add_argument("--serve-max-active", type=int, default=8)
     ap.add_argument("--profile", action="store_true",
@@ -279,8 +319,14 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel="k_stencil<double, kApplyFused> (h = z + gamma h_prev, every other launch the "
-                           "two pending u steps, Ah = K h, h.Ah partials; launches timed alternately)",
+                           "two pending u steps, Ah = K h, h.Ah partials)",
                     avg_launch_us=round(fp64["apply_ms"] * 1e3, 2),
+                    launch_time_source=f"HIP events on the solver's stream around the {fp64['apply_launches']} "
+                                       "fused applies of the timed PCG loops",
+                    replay_avg_launch_us=round(fp64["replay_ms"] * 1e3, 2),
+                    traffic_source=("not measured in this run: FETCH_SIZE + WRITE_SIZE per launch from the "
+                                    "rocprofv3 --pmc passes summarised in profiles/stencil_apply_cube200_pmc.txt")
+                    if traffic else None,
                     algorithmic_bytes_per_launch=int(alg_bytes))
 
     out = {
@@ -319,9 +365,9 @@ def main():
             "latency_ms_p50": float(np.percentile(mwalls, 50) * 1e3),
             "latency_ms_p99": float(np.percentile(mwalls, 99) * 1e3),
             "lstep": mx["lsteps"][0],
-            "utot_rel_err_vs_fp64": abs(mx["utot"] - fp64["utot"]) / abs(fp64["utot"]),
-            "stress_xx_rel_err_vs_fp64": abs(mx["stress_xx"] - fp64["stress_xx"]) / abs(fp64["stress_xx"]),
+            "errors_vs_fp64": output_errors(mx, fp64),
             "apply_us": round(mx["apply_ms"] * 1e3, 2),
+            "apply_replay_us": round(mx["replay_ms"] * 1e3, 2),
         }
     if not args.no_six:
         six = {}
@@ -341,7 +387,9 @@ def main():
         six["mixed_six_vs_fp64_single_solves_ratio"] = six["mixed"]["solves_per_s"] / out["solves_per_s"]
         out["six_rhs"] = six
     if args.serve_requests > 0:
-        out["serving"] = run_serving(args.serve_requests, args.serve_interarrival_ms, args.serve_max_active)
+        reqs = serving_requests(args.serve_requests)
+        out["serving"] = run_serving(reqs, args.serve_interarrival_ms, args.serve_max_active)
+        del reqs
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, c11, c12, c44, e6, args.cpu_iters)
     else:

@@ -53,6 +53,8 @@ void release(Solver &s)
     if (s.st_host) (void)hipHostFree(s.st_host);
     for (hipEvent_t &e : s.ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : s.apply_ev)
+        if (e) (void)hipEventDestroy(e);
     if (s.stream) (void)hipStreamDestroy(s.stream);
 }
 
@@ -376,6 +378,12 @@ int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const doubl
         if (int rc = ensure_mixed_buffers(s)) return rc;
     if (int rc = ensure_hist(s, ldemb)) return rc;
     const int64_t nv = (int64_t)s.nrhs * 3 * s.ns;
+    if (s.time_applies && s.apply_ev.empty()) {
+        s.apply_ev.assign(2 * kApplyTimed, nullptr);
+        s.apply_it.assign(kApplyTimed, 0);
+        for (hipEvent_t &e : s.apply_ev) E3X_HIP(hipEventCreate(&e));
+    }
+    s.apply_n = 0;
 
     E3X_HIP(hipEventRecord(s.ev[0], s.stream));
     // block-Jacobi inverse, built once per femat (f90:1498-1520)
@@ -539,6 +547,23 @@ int e3x_dembx_end(e3x_solver *h, int *lstep, double *gg, double *history, int *e
     (void)hipEventElapsedTime(&ms1, s.ev[1], s.ev[2]);
     s.last_precond_ms = ms0;
     s.last_pcg_ms = ms1;
+    // mean of the timed applies that did work: iterations up to the last stop (launches
+    // past it find the stop state and leave at once)
+    int last_stop = 0;
+    for (int r = 0; r < s.nrhs; r++) last_stop = std::max(last_stop, s.st_host[r].stop_iter);
+    double sum = 0;
+    int n = 0;
+    for (int k = 0; k < s.apply_n; k++) {
+        if (s.apply_it[k] > last_stop) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, s.apply_ev[2 * k], s.apply_ev[2 * k + 1]) == hipSuccess) {
+            sum += ms;
+            n++;
+        }
+    }
+    s.last_apply_n = n;
+    s.last_apply_ms = n > 0 ? sum / n : 0.0;
+    s.apply_n = 0;
     for (int r = 0; r < s.nrhs; r++) {
         const PcgState &st = s.st_host[r];
         const int L = st.stop_iter > 0 ? st.stop_iter : 0;
@@ -745,6 +770,22 @@ int e3x_last_timing(e3x_solver *h, double *precond_ms, double *pcg_ms)
     Solver &s = S(h);
     if (precond_ms) *precond_ms = s.last_precond_ms;
     if (pcg_ms) *pcg_ms = s.last_pcg_ms;
+    return E3X_OK;
+}
+
+int e3x_set_apply_timing(e3x_solver *h, int on)
+{
+    if (int rc = check_idle(h)) return rc;
+    S(h).time_applies = on != 0;
+    return E3X_OK;
+}
+
+int e3x_last_apply_timing(e3x_solver *h, int *launches, double *avg_ms)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    if (launches) *launches = s.last_apply_n;
+    if (avg_ms) *avg_ms = s.last_apply_ms;
     return E3X_OK;
 }
 

@@ -11,6 +11,7 @@
 namespace e3x {
 
 constexpr int kMaxRhs = 6;
+constexpr int kApplyTimed = 1024;  // fused applies timed per PCG (e3x_set_apply_timing)
 constexpr int kBlock = 256;        // threads per block of every kernel
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
 
@@ -141,6 +142,14 @@ struct Solver {
     int pcg_variant = 0;  // e3x_pcg_variant of the next PCG
     bool precond_identity = false;  // minv (minv32) hold the identity (E3X_PCG_PLAIN_CG)
 
+    // e3x_set_apply_timing: HIP events around the fused applies (iterations >= 2) of the
+    // following PCG runs, on the solver's stream; pair k brackets iteration apply_it[k]
+    bool time_applies = false;
+    std::vector<hipEvent_t> apply_ev;  // 2 * kApplyTimed, created on first use
+    std::vector<int> apply_it;
+    int apply_n = 0;                   // pairs recorded in the PCG in progress
+    int last_apply_n = 0;
+    double last_apply_ms = 0;          // mean over the launches of the last PCG
     bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
     double last_precond_ms = 0, last_pcg_ms = 0;
     int64_t bytes = 0;

@@ -630,12 +630,23 @@ hipError_t launch_apply_hah(Solver &s, bool mixed, int it, int usteps)
     for (int r = 0; r < s.nrhs; r++)
         if (!((s.pcg_known_stopped >> r) & 1u)) rmap[rows++] = r;
     if (rows == 0) return hipSuccess;  // every right-hand side has stopped: nothing to apply
+    const bool timed = s.time_applies && it >= 2 && s.apply_n < kApplyTimed &&
+                       (int)s.apply_ev.size() == 2 * kApplyTimed;
+    if (timed) {
+        hipError_t e = hipEventRecord(s.apply_ev[2 * s.apply_n], s.stream);
+        if (e != hipSuccess) return e;
+    }
     if (mixed) {
         float *hb[2] = {s.h32_alt, s.h32};
         apply_iter<float>(s, s.dk32, s.sten32, hb, s.z32, s.ah32, it, usteps, rmap, rows);
     } else {
         double *hb[2] = {s.h_alt, s.h};
         apply_iter<double>(s, s.dk, s.sten, hb, s.z, s.ah, it, usteps, rmap, rows);
+    }
+    if (timed) {
+        hipError_t e = hipEventRecord(s.apply_ev[2 * s.apply_n + 1], s.stream);
+        if (e != hipSuccess) return e;
+        s.apply_it[s.apply_n++] = it;
     }
     return launch_hah_finish(s, (int)stencil_parts(s), it);
 }

@@ -217,6 +217,17 @@ class Elas3D:
         L.check(self._lib.e3x_time_apply(self._h, precision, reps, C.byref(ms)))
         return ms.value
 
+    def set_apply_timing(self, on=True):
+        """Time the stencil applies of the following PCG runs in the loop itself (HIP
+        events on the solver's stream; e3x_set_apply_timing)."""
+        L.check(self._lib.e3x_set_apply_timing(self._h, 1 if on else 0))
+
+    def last_apply_timing(self):
+        """-> (launches timed, mean ms) of the last PCG's fused applies."""
+        n, ms = C.c_int(), C.c_double()
+        L.check(self._lib.e3x_last_apply_timing(self._h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
     def tables(self):
         cm = np.zeros(36 * self.nphase)
         dk = np.zeros(576 * self.nphase)

@@ -269,6 +269,17 @@ int e3x_get_tables(e3x_solver *s, double *cmod, double *dk);
 int e3x_last_timing(e3x_solver *s, double *precond_ms, double *pcg_ms);
 
 /*
+ * Measurement hook (no counterpart in the reference): with on != 0, the handle's
+ * following e3x_dembx / e3x_dembx_ex / e3x_dembx_begin runs record HIP events on the
+ * solver's stream around the stencil-apply launch of every iteration from the second on
+ * (the fused form; at most the first 1024 of a run), and e3x_last_apply_timing returns
+ * how many of those launches did work (iterations up to the last stop) and their mean
+ * duration in ms: the apply as the PCG loop runs it, between its neighbours in the loop.
+ */
+int e3x_set_apply_timing(e3x_solver *s, int on);
+int e3x_last_apply_timing(e3x_solver *s, int *launches, double *avg_ms);
+
+/*
  * Measurement hook (no counterpart in the reference): replay `reps` launches of the
  * PCG's stencil-apply kernel in the form the loop runs from its second iteration on
  * (direction h = z + gamma h_prev, u -= lambda h_prev, Ah = K h with the h.Ah

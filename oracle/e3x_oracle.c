@@ -577,6 +577,11 @@ int oc_dembx(int nx, int ny, int nz, const int32_t *pix, const double *dk, doubl
  * h = gb at entry, cg:949), the divergence reset (src only) and the order of the stop
  * test and the direction update (src: update first, f90:1996/2007; sweep and plain CG:
  * test first, A_2_0:1518-1526, cg:1050-1056). */
+/* Wall seconds of the iteration loop (f90:1574-2012) of the last oc_dembx_variant call,
+ * setup (preconditioner, z, h) excluded: the CPU baseline's timing scope. */
+static double g_loop_seconds = 0.0;
+double oc_last_loop_seconds(void) { return g_loop_seconds; }
+
 int oc_dembx_variant(int nx, int ny, int nz, const int32_t *pix, const double *dk, double *u,
                      double *gb, double *h_io, double gg0, float tol, int ldemb, int *lstep,
                      double *gg_out, double *history, int *reset_count_out, int variant)
@@ -600,6 +605,7 @@ int oc_dembx_variant(int nx, int ny, int nz, const int32_t *pix, const double *d
     int reset_count = 0, L = 0, status = OC_MAXITER;
     (void)rel_res_min;
 
+    const double t_loop = omp_get_wtime();
     for (int ijk = 1; ijk <= ldemb; ijk++) {
         L = L + 1;
         oc_apply(nx, ny, nz, pix, dk, h, ah);                 /* f90:1578-1883 */
@@ -631,6 +637,7 @@ int oc_dembx_variant(int nx, int ny, int nz, const int32_t *pix, const double *d
         rel_res_prev = rel_res;                               /* f90:2004 */
         if (rel_res < tol_d) { status = OC_CONVERGED; break; } /* f90:2007 (reached by src only) */
     }
+    g_loop_seconds = omp_get_wtime() - t_loop;
     *lstep = L;
     *gg_out = gg;
     if (reset_count_out) *reset_count_out = reset_count;

@@ -78,6 +78,8 @@ int oc_dembx(int nx, int ny, int nz, const int32_t *pix, const double *dk, doubl
  * src/elas3dxtal_pcg.f90 (f90:1356-2043), applications/sec3_2/A_x_y/elas3dxtal_pcg.f90
  * (A_2_0:1045-1558) and applications/sec3_1/cg/n_N/elas3dxtal_cg.f90 (cg:915-1074). */
 enum { OC_PCG_REFERENCE = 0, OC_PCG_SWEEP = 1, OC_PCG_PLAIN_CG = 2 };
+/* Wall seconds of the last oc_dembx_variant loop (setup excluded). */
+double oc_last_loop_seconds(void);
 int oc_dembx_variant(int nx, int ny, int nz, const int32_t *pix, const double *dk, double *u,
                      double *gb, double *h, double gg0, float tol, int ldemb, int *lstep, double *gg,
                      double *history, int *reset_count, int variant);

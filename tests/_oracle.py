@@ -57,6 +57,7 @@ def _load():
                          C.c_void_p, C.c_void_p, C.POINTER(OcResult)]),
         "oc_set_threads": (v, [i]),
         "oc_get_threads": (i, []),
+        "oc_last_loop_seconds": (d, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -561,14 +561,40 @@ def test_multi_rhs_stops_in_later_chunks(precision):
         assert np.array_equal(u_b[r], u1)
         assert np.array_equal(batch.history[r], one.history[0])
         if precision == L.PREC_FP64:
-            ro = oc.solve(shape, 31, pix, rod, c11, C12, C44, strains[r], tol=1e-10)
-            K = int(batch.lstep[r])
-            assert abs(K - ro["lstep"]) <= max(1, int(0.01 * ro["lstep"]))
-            if K != ro["lstep"]:
-                ro = oc.solve(shape, 31, pix, rod, c11, C12, C44, strains[r], tol=0.0, ldemb=K)
-            assert _rel(u_b[r], ro["u"]) < 1e-10
-            assert abs(batch.utot[r] - ro["utot"]) < 1e-10 * abs(ro["utot"])
-            assert _rel(batch.history[r], ro["history"]) < 1e-8
+            _check_deep_fp64_against_oracle(shape, pix, rod, c11, strains[r], int(batch.lstep[r]), u_b[r],
+                                            batch.history[r], batch.utot[r])
+
+
+def _check_deep_fp64_against_oracle(shape, pix, rod, c11, e6, K, u, history, utot):
+    """FP64 iterate at stop iteration K against the oracle's at K, for solves driven far
+    below the reference's tolerance.  There the PCG amplifies rounding: started from its
+    own initial gradient perturbed by one ulp, the oracle's iterate after ~120 iterations
+    of the A = 5 case moves by ~2e-8 and its residual history by ~2e-2.  So the bar is ten
+    times the oracle's own one-ulp spread at K, and never looser than the usual 1e-10 (u)
+    and 1e-8 (history)."""
+    nphase = len(rod) + 1
+    dk_o = oc.dk(oc.cmod(nphase, oc.rotate_stiffness(rod, c11, C12, C44), [0.0, 0.0]))
+    b_o, c_o = oc.femat_rhs(shape, pix, dk_o, e6)
+    u0 = oc.affine(shape, e6)
+    g0, _ = oc.energy(shape, pix, dk_o, b_o, c_o, u0)
+    gg0 = oc.sumsq(g0)
+    ro = oc.dembx(shape, pix, dk_o, u0, g0, gg0, tol=1e-10)
+    assert abs(K - ro["lstep"]) <= max(1, int(0.01 * ro["lstep"]))
+    if K != ro["lstep"]:
+        ro = oc.dembx(shape, pix, dk_o, u0, g0, gg0, tol=0.0, ldemb=K)
+    rng = np.random.default_rng(K)
+    g1 = g0 * (1.0 + 2.0 ** -53 * rng.choice([-1.0, 1.0], g0.size))
+    rp = oc.dembx(shape, pix, dk_o, u0, g1, gg0, tol=0.0, ldemb=K)
+    spread_u, spread_h = _rel(rp["u"], ro["u"]), _rel(rp["history"], ro["history"])
+    _, ut_o = oc.energy(shape, pix, dk_o, b_o, c_o, ro["u"])
+    _, ut_p = oc.energy(shape, pix, dk_o, b_o, c_o, rp["u"])
+    spread_e = abs(ut_p - ut_o) / abs(ut_o)
+    err_u, err_h, err_e = _rel(u, ro["u"]), _rel(history, ro["history"]), abs(utot - ut_o) / abs(ut_o)
+    print(f"K {K}: u {err_u:.2e} (oracle one-ulp spread {spread_u:.2e}), history {err_h:.2e} ({spread_h:.2e}), "
+          f"energy {err_e:.2e} ({spread_e:.2e})")
+    assert err_u < max(1e-10, 10 * spread_u)
+    assert err_h < max(1e-8, 10 * spread_h)
+    assert err_e < max(1e-10, 10 * spread_e)
 
 
 @pytest.mark.parametrize("precision,tol", [(L.PREC_FP64, 1e-10), (L.PREC_MIXED, 1e-6)])
