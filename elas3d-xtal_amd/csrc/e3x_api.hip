@@ -773,6 +773,12 @@ int e3x_last_timing(e3x_solver *h, double *precond_ms, double *pcg_ms)
     return E3X_OK;
 }
 
+int e3x_stress_dev(int nx, int ny, int nz, int nphase, int nrhs, const int32_t *pix, const double *cmod,
+                   const double *strain6, const double *u, double *averages, void *stream)
+{
+    return stress_dev(nx, ny, nz, nphase, nrhs, pix, cmod, strain6, u, averages, (hipStream_t)stream);
+}
+
 int e3x_set_apply_timing(e3x_solver *h, int on)
 {
     if (int rc = check_idle(h)) return rc;

@@ -180,6 +180,9 @@ hipError_t launch_pcg_finalize(Solver &s, bool mixed, int replace_every);
 // e3x_post.hip
 hipError_t launch_stress(Solver &s);                          // results[0..12*nrhs)
 hipError_t launch_fullfield(Solver &s, int rhs, double *stress_dev, double *vm_dev);
+// e3x_stress_dev (caller-owned device arrays, scratch in e3x_post.hip); an e3x_status
+int stress_dev(int nx, int ny, int nz, int nphase, int nrhs, const int32_t *pix, const double *cmod,
+               const double *strain6, const double *u, double *averages, hipStream_t st);
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
 hipError_t launch_to_f64(const float *src, double *dst, int64_t n, hipStream_t st);
 // one right-hand side's field: component planes [3][ns] <-> node-interleaved [ns][3]

@@ -3,6 +3,8 @@
 #include "e3x_device.hpp"
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace e3x {
 namespace {
@@ -13,11 +15,15 @@ namespace {
 struct PostArgs {
     int nx, ny, nz, nrhs;
     int64_t ns;
-    const uint16_t *pix;
-    const double *cmod, *u;
+    const uint16_t *pix;     // 0-based phases (a handle's copy)
+    const int32_t *pix32;    // DEV form: the caller's labels 1..nphase
+    int nphase;              // DEV form: labels outside [1, nphase] give NaN
+    const double *cmod, *u;  // u: [nrhs][ns][3], DEV form [nrhs][3][ns] (u(ns,3) per rhs)
     double e[6 * kMaxRhs];
 };
 
+// DEV: the caller's arrays as the reference lays them out (int32 labels, u(ns,3)).
+template <bool DEV>
 __device__ void voxel_strain_stress(const PostArgs &A, int r, int64_t m, double eps[6], double sig[6])
 {
     const int i = (int)(m % A.nx), j = (int)((m / A.nx) % A.ny), k = (int)(m / ((int64_t)A.nx * A.ny));
@@ -26,7 +32,7 @@ __device__ void voxel_strain_stress(const PostArgs &A, int r, int64_t m, double 
     double uu[8][3];
     for (int a = 0; a < 8; a++) {
         const int64_t q = wrap(i + PX(a), A.nx) + (int64_t)A.nx * (wrap(j + PY(a), A.ny) + (int64_t)A.ny * wrap(k + PZ(a), A.nz));
-        for (int c = 0; c < 3; c++) uu[a][c] = u[3 * q + c];
+        for (int c = 0; c < 3; c++) uu[a][c] = DEV ? u[c * A.ns + q] : u[3 * q + c];
     }
     if (i == A.nx - 1)
         for (int a = 0; a < 8; a++)
@@ -46,7 +52,17 @@ __device__ void voxel_strain_stress(const PostArgs &A, int r, int64_t m, double 
             else if (n3 == 1) { eps[1] += dy * v; eps[4] += dz * v; eps[5] += dx * v; }
             else { eps[2] += dz * v; eps[3] += dx * v; eps[4] += dy * v; }
         }
-    const double *c = A.cmod + (int64_t)A.pix[m] * 36;
+    int p;
+    if constexpr (DEV) {
+        p = A.pix32[m] - 1;
+        if (p < 0 || p >= A.nphase) {  // an unchecked caller label: no table read, NaN out
+            for (int q = 0; q < 6; q++) sig[q] = __builtin_nan("");
+            return;
+        }
+    } else {
+        p = A.pix[m];
+    }
+    const double *c = A.cmod + (int64_t)p * 36;
     for (int q = 0; q < 6; q++) {
         double s = 0.0;
         for (int t = 0; t < 6; t++) s += c[q * 6 + t] * eps[t];
@@ -54,7 +70,7 @@ __device__ void voxel_strain_stress(const PostArgs &A, int r, int64_t m, double 
     }
 }
 
-template <int NR>
+template <int NR, bool DEV = false>
 __global__ __launch_bounds__(kBlock) void k_stress(PostArgs A, double *partials)
 {
     double part[12 * NR];
@@ -64,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(PostArgs A, double *partials)
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             double eps[6], sig[6];
-            voxel_strain_stress(A, r, m, eps, sig);
+            voxel_strain_stress<DEV>(A, r, m, eps, sig);
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 part[12 * r + q] += sig[q];
@@ -90,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_fullfield(PostArgs A, int r, double 
     const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (m >= A.ns) return;
     double eps[6], s[6];
-    voxel_strain_stress(A, r, m, eps, s);
+    voxel_strain_stress<false>(A, r, m, eps, s);
     if (sf)
         for (int q = 0; q < 6; q++) sf[(int64_t)q * A.ns + m] = s[q];
     if (vm)
@@ -149,7 +165,80 @@ void stress_nr(Solver &s)
                        s.results);
 }
 
+// e3x_stress_dev: the averages of caller-owned device arrays, with the block partials in
+// this file's scratch array (the grid of launch_stress, so the sums run in the same order).
+template <int NR>
+void stress_dev_nr(const PostArgs &A, int g, double *scratch, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_stress<NR, true>), dim3(g), dim3(kBlock), 0, st, A, scratch);
+    hipLaunchKernelGGL(k_stress_finish<NR>, dim3(1), dim3(kFinBlock), 0, st, (const double *)scratch, g, A.ns, out);
+}
+
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of e3x_stress_dev: the block partials of the averages, 12 per right-hand side
+// per block of the streaming grid (at most kVecGridMax blocks), in device memory the
+// library holds for the life of the process on every GPU it runs on (1.2 MB).  Host code
+// reaches a device's copy only through the address hipGetSymbolAddress returns for that
+// device, never through the array's name.
+constexpr int64_t kStressScratch = 12 * (int64_t)kMaxRhs * kVecGridMax;
+static __device__ __align__(256) double scratch[kStressScratch];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kScratchDevices] = {};
+
+// This device's copy of the scratch array (looked up once per device, with it current).
+hipError_t scratch_for(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+int stress_dev(int nx, int ny, int nz, int nphase, int nrhs, const int32_t *pix, const double *cmod,
+               const double *strain6, const double *u, double *averages, hipStream_t st)
+{
+    if (nx < 2 || ny < 2 || nz < 2) return report_error(1, "grid extents must be >= 2");
+    if ((int64_t)nx * ny * nz > (int64_t)INT32_MAX) return report_error(1, "grid exceeds 2^31 nodes");
+    if (nphase < 1) return report_error(1, "nphase must be >= 1");
+    if (nrhs < 1 || nrhs > kMaxRhs)
+        return report_error(1, "nrhs must be in [1, 6]: the scratch holds the partials of at most 6 right-hand sides");
+    if (!pix || !cmod || !strain6 || !u || !averages) return report_error(1, "NULL argument");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices)
+        return report_error(2, "no current HIP device (or device index beyond 63)");
+    double *sc = nullptr;
+    if (hipError_t e = scratch_for(dev, &sc); e != hipSuccess)
+        return report_error(2, (std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)).c_str());
+    PostArgs A{};
+    A.nx = nx; A.ny = ny; A.nz = nz; A.nrhs = nrhs; A.ns = (int64_t)nx * ny * nz;
+    A.pix32 = pix; A.nphase = nphase; A.cmod = cmod; A.u = u;
+    for (int q = 0; q < 6 * nrhs; q++) A.e[q] = strain6[q];
+    const int64_t vb = (A.ns + kBlock - 1) / kBlock;
+    const int g = (int)(vb < kVecGridMax ? vb : kVecGridMax);  // vector_grid(): launch_stress's grid
+    static_assert(kStressScratch >= 12 * (int64_t)kMaxRhs * kVecGridMax, "scratch too small");
+    switch (nrhs) {
+    case 1: stress_dev_nr<1>(A, g, sc, averages, st); break;
+    case 2: stress_dev_nr<2>(A, g, sc, averages, st); break;
+    case 3: stress_dev_nr<3>(A, g, sc, averages, st); break;
+    case 4: stress_dev_nr<4>(A, g, sc, averages, st); break;
+    case 5: stress_dev_nr<5>(A, g, sc, averages, st); break;
+    default: stress_dev_nr<6>(A, g, sc, averages, st); break;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return report_error(2, (std::string("stress launch: ") + hipGetErrorString(e)).c_str());
+    return 0;
+}
 
 hipError_t launch_stress(Solver &s)
 {

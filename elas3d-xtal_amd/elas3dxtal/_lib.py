@@ -64,6 +64,7 @@ SIGNATURES = {
     "e3x_get_tables": (C.c_int, [_vp, _dp, _dp]),
     "e3x_last_timing": (C.c_int, [_vp, _dp, _dp]),
     "e3x_time_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "e3x_stress_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _dp, _vp, _vp, _vp]),
     "e3x_set_apply_timing": (C.c_int, [_vp, C.c_int]),
     "e3x_last_apply_timing": (C.c_int, [_vp, _ip, _dp]),
 }

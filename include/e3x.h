@@ -236,6 +236,27 @@ int e3x_dembx_end(e3x_solver *s, int *lstep, double *gg, double *history, int *e
 int e3x_stress(e3x_solver *s, double *stress6, double *strain6);
 
 /*
+ * e3x_stress on caller-owned device memory, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  Enqueues on `stream` (a hipStream_t;
+ * NULL: the null stream) of the calling thread's current device the averages of
+ * stress_OpenMP (f90:2149-2408) and returns without waiting:
+ *   pix       device int32[ns], labels 1..nphase (not checked on the host; a voxel with a
+ *             label outside [1, nphase] makes the averages NaN)
+ *   cmod      device double[nphase*36] (cmod(p,i,j) row-major per phase, e3x_get_tables)
+ *   strain6   host double[6*nrhs], the applied strains as e3x_femat takes them
+ *   u         device double[nrhs*3*ns], u(ns,3) per right-hand side (component planes)
+ *   averages  device double[nrhs*12]: per right-hand side stress6 then strain6, as
+ *             e3x_stress returns them (the same kernel and summation order)
+ * The block partials go to a scratch array of 12*6*2048 doubles (1.2 MB) that the library
+ * holds on each GPU it runs on until the process ends: nrhs is at most 6 (E3X_EINVAL
+ * otherwise).  Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (other streams, other host threads) race, so serialise them.
+ */
+int e3x_stress_dev(int nx, int ny, int nz, int nphase, int nrhs, const int32_t *pix, const double *cmod,
+                   const double *strain6, const double *u, double *averages, void *stream);
+
+/*
  * stress_fullfield_OpenMP computation part (f90:2639-2833) for right-hand side
  * `rhs`: stress[6][ns] (Fortran stress_field(ns,6)) and von Mises vm[ns]; either
  * may be NULL.  The HDF5 write (f90:2840-2874) is e3x_h5_write_fullfield.
