@@ -151,7 +151,7 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     *out = nullptr;
     if (nx < 2 || ny < 2 || nz < 2) return fail(E3X_EINVAL, "grid extents must be >= 2");
     if ((int64_t)nx * ny * nz > (int64_t)INT32_MAX) return fail(E3X_EINVAL, "grid exceeds 2^31 nodes");
-    if (nphase < 1 || nphase > 65535) return fail(E3X_EINVAL, "nphase must be in [1, 65535]");
+    if (nphase < 1 || nphase > kMaxPhases) return fail(E3X_EINVAL, "nphase must be in [1, 2^31 - 2]");
     if (nrhs < 1 || nrhs > kMaxRhs) return fail(E3X_EINVAL, "nrhs must be in [1, 6]");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(E3X_EDEVICE, "no HIP device");
@@ -159,6 +159,7 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     Solver *s = new (std::nothrow) Solver();
     if (!s) return fail(E3X_ENOMEM, "host allocation failed");
     s->nx = nx; s->ny = ny; s->nz = nz; s->nphase = nphase; s->nphase_cap = nphase; s->nrhs = nrhs;
+    s->wide = nphase > kNarrowPhases;
     s->device = device;
     s->ns = (int64_t)nx * ny * nz;
     auto bail = [&](int code) {
@@ -183,7 +184,8 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->dk32, (int64_t)nphase * 576));
     CK(dalloc(*s, &s->sten32, (int64_t)nphase * kStenHalf));
     CK(dalloc(*s, &s->orient, (int64_t)3 * std::max(nphase - 1, 1)));
-    CK(dalloc(*s, &s->pix, ns));
+    const int64_t id_bytes = s->wide ? 4 : 2;  // phase-id width
+    CK(dalloc(*s, reinterpret_cast<uint8_t **>(&s->pix), id_bytes * ns));
     CK(dalloc(*s, &s->perm, stencil_blocks(*s) * kSB));
     CK(dalloc(*s, &s->u, nv));
     CK(dalloc(*s, &s->g, nv));
@@ -193,7 +195,7 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->z, nv));
     CK(dalloc(*s, &s->ah, nv));
     CK(dalloc(*s, &s->minv, 6 * ns));
-    CK(dalloc(*s, &s->hphase, ns));
+    CK(dalloc(*s, reinterpret_cast<uint8_t **>(&s->hphase), id_bytes * ns));
     CK(dalloc(*s, &s->minv_hom, (int64_t)nphase * 6));
     CK(dalloc(*s, &s->minv_hom32, (int64_t)nphase * 6));
     CK(dalloc(*s, &s->cconst, kMaxRhs));
@@ -242,7 +244,6 @@ int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orie
     if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
     if (!pix) return fail(E3X_EINVAL, "pix is NULL");
-    std::vector<uint16_t> p16((size_t)s.ns);
     int32_t lo = INT32_MAX, hi = INT32_MIN;
     for (int64_t m = 0; m < s.ns; m++) {
         lo = std::min(lo, pix[m]);
@@ -253,12 +254,20 @@ int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orie
         return fail(E3X_EPHASE, "phase label in pix out of bounds: min " + std::to_string(lo) + ", max " +
                                     std::to_string(hi) + ", nphase " + std::to_string(s.nphase));
     s.counts.assign((size_t)s.nphase, 0);
-    for (int64_t m = 0; m < s.ns; m++) {
-        p16[(size_t)m] = (uint16_t)(pix[m] - 1);
-        s.counts[(size_t)(pix[m] - 1)]++;
-    }
+    for (int64_t m = 0; m < s.ns; m++) s.counts[(size_t)(pix[m] - 1)]++;
     E3X_HIP(hipSetDevice(s.device));
-    E3X_HIP(hipMemcpyAsync(s.pix, p16.data(), sizeof(uint16_t) * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
+    // 0-based ids in the handle's width (16 bits up to kNarrowPhases phases, else 32)
+    std::vector<uint16_t> p16;
+    std::vector<uint32_t> p32;
+    if (s.wide) {
+        p32.resize((size_t)s.ns);
+        for (int64_t m = 0; m < s.ns; m++) p32[(size_t)m] = (uint32_t)(pix[m] - 1);
+        E3X_HIP(hipMemcpyAsync(s.pix, p32.data(), sizeof(uint32_t) * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
+    } else {
+        p16.resize((size_t)s.ns);
+        for (int64_t m = 0; m < s.ns; m++) p16[(size_t)m] = (uint16_t)(pix[m] - 1);
+        E3X_HIP(hipMemcpyAsync(s.pix, p16.data(), sizeof(uint16_t) * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
+    }
     if (orientation && s.nphase > 1)
         E3X_HIP(hipMemcpyAsync(s.orient, orientation, sizeof(double) * 3 * (size_t)(s.nphase - 1),
                                hipMemcpyHostToDevice, s.stream));

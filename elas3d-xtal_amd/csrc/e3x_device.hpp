@@ -14,6 +14,22 @@ __host__ __device__ constexpr int PZ(int a) { return a >> 2; }
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// Phase of voxel m (0-based): the ids are uint16, or uint32 on a wide handle (more than
+// kNarrowPhases phases).  `wide` is uniform over a launch.
+__device__ __forceinline__ int phase_at(const void *pix, int wide, int64_t m)
+{
+    return wide ? (int)static_cast<const uint32_t *>(pix)[m] : (int)static_cast<const uint16_t *>(pix)[m];
+}
+
+// Homogeneous phase of node m, or -1 when its 8 voxels hold several phases (the all-ones
+// id in either width).
+__device__ __forceinline__ int hom_phase_at(const void *hphase, int wide, int64_t m)
+{
+    if (wide) return (int)static_cast<const uint32_t *>(hphase)[m];
+    const int v = static_cast<const uint16_t *>(hphase)[m];
+    return v == 0xFFFF ? -1 : v;
+}
+
 // Consecutive tile ids on one XCD: the hardware hands workgroup b to XCD b % 8, so
 // neighbouring tiles (which share halo lines) land in the same L2.
 __device__ __forceinline__ int xcd_remap(int bid, int nb)

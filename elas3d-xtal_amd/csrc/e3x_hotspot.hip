@@ -24,7 +24,8 @@ constexpr int kHotRadiusMax = 3;  // sigma <= 1.5 (radius ceil(2 sigma))
 struct HotArgs {
     int nx, ny, nz, nsolid, r;
     int64_t ns;
-    const uint16_t *pix;  // 0-based phase per voxel
+    const void *pix;  // 0-based phase id per voxel, uint16 or (wide) uint32
+    int wide;
     const double *vm;
     float *sm;            // smoothed field
     double w[kHotRadiusMax + 1];   // normalised 1-D weights at |d| = 0..r
@@ -34,7 +35,7 @@ struct HotArgs {
     unsigned long long *count;  // [0] strict maxima, [1] voxels with an equal neighbour
 };
 
-__device__ __forceinline__ bool solid(const HotArgs &A, int64_t m) { return (int)A.pix[m] + 1 <= A.nsolid; }
+__device__ __forceinline__ bool solid(const HotArgs &A, int64_t m) { return phase_at(A.pix, A.wide, m) + 1 <= A.nsolid; }
 
 // s = G*(vm . solid) / G*(solid) on solid voxels, 0 elsewhere (.m:86-97).  The numerator
 // is summed in double, the denominator in single, and the quotient taken in single, as
@@ -160,6 +161,7 @@ hipError_t launch_hotspots(Solver &s, int nsolid, double sigma, float thr_mpa, i
     HotArgs A{};
     A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.ns = ns; A.nsolid = nsolid;
     A.pix = s.pix;
+    A.wide = s.wide ? 1 : 0;
     A.vm = s.post + 6 * ns;
     A.sm = reinterpret_cast<float *>(s.post);
     A.cand = reinterpret_cast<HotEntry *>(s.post + ns);       // planes 1-2: ns entries of 16 B

@@ -11,6 +11,11 @@
 namespace e3x {
 
 constexpr int kMaxRhs = 6;
+// Phase ids are stored in 16 bits up to this many phases, in 32 bits above (the
+// reference's anisotropy-sweep inputs have 81,488 grains); the all-ones id marks a node
+// whose voxels hold several phases, so wide ids stay below 2^31 - 1.
+constexpr int kNarrowPhases = 65535;
+constexpr int kMaxPhases = 0x7FFFFFFE;
 constexpr int kApplyTimed = 1024;  // fused applies timed per PCG (e3x_set_apply_timing)
 constexpr int kBlock = 256;        // threads per block of every kernel
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
@@ -83,7 +88,9 @@ struct Solver {
     double *cmod = nullptr, *dk = nullptr, *sten = nullptr;  // [p][36], [p][576], [p][kStenHalf]
     float *dk32 = nullptr, *sten32 = nullptr;
     double *orient = nullptr;                                  // [3*(nphase-1)]
-    uint16_t *pix = nullptr;                                   // 0-based phase per voxel
+    // 0-based phase per voxel: uint16 with at most kNarrowPhases phases, else uint32 (wide)
+    void *pix = nullptr;
+    bool wide = false;
     // per stencil tile, the order in which the block's lanes take its nodes (node t of
     // the tile's wave-cube numbering): homogeneous nodes by phase, then mixed nodes by
     // their phases, so that waves hold few distinct phases; built on upload
@@ -92,9 +99,10 @@ struct Solver {
     // fields, [nrhs][ns][3] (the three components of a node side by side)
     double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;
     double *minv = nullptr;                                    // [ns][6]
-    // homogeneous phase of each node (its 8 voxels share it), 0xFFFF for the others, and
-    // the M^-1 block of a homogeneous node per phase ([nphase][6], FP64 and FP32)
-    uint16_t *hphase = nullptr;
+    // homogeneous phase of each node (its 8 voxels share it), all ones for the others (the
+    // width of pix), and the M^-1 block of a homogeneous node per phase ([nphase][6], FP64
+    // and FP32)
+    void *hphase = nullptr;
     double *minv_hom = nullptr;
     float *minv_hom32 = nullptr;
     float *g32 = nullptr, *h32 = nullptr, *z32 = nullptr, *ah32 = nullptr, *minv32 = nullptr;

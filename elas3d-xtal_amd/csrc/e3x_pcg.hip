@@ -41,7 +41,8 @@ struct PcgArgs {
     T *g, *h, *z, *ah;
     const T *hm1;   // h_{it-1}
     const T *minv;
-    const uint16_t *hphase;  // homogeneous phase of each node, 0xFFFF for the others
+    const void *hphase;      // homogeneous phase of each node, all ones for the others
+    int wide;                // hphase holds uint32 ids (else uint16)
     const T *minv_hom;       // [nphase][6]: M^-1 of a homogeneous node of each phase
     double *partials;
     PcgState *st;
@@ -179,8 +180,8 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
         // for a homogeneous node (the same bits as its entry in minv), else from minv
         double mi[6];
         if constexpr (DO_Z) {
-            const int hp = A.hphase[m];
-            if (hp != 0xFFFF) {
+            const int hp = hom_phase_at(A.hphase, A.wide, m);
+            if (hp >= 0) {
 #pragma unroll
                 for (int q = 0; q < 6; q++) mi[q] = (double)A.minv_hom[6 * hp + q];
             } else {
@@ -342,6 +343,7 @@ PcgArgs<T> pcg_args(const Solver &s, int it)
     A.ns = s.ns; A.it = it; A.u = s.u;
     A.partials = s.partials; A.st = s.st; A.hist = s.hist;
     A.hphase = s.hphase;
+    A.wide = s.wide ? 1 : 0;
     if constexpr (sizeof(T) == 4) {
         A.g = s.g32; A.h = s.h32; A.z = s.z32; A.ah = s.ah32; A.minv = s.minv32; A.minv_hom = s.minv_hom32;
     } else {

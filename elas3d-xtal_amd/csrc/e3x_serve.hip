@@ -81,9 +81,10 @@ int64_t mixed_bytes(int nx, int ny, int nz)
 int64_t footprint(int nx, int ny, int nz, int nphase_cap, bool mixed, int ldemb)
 {
     const int64_t ns = (int64_t)nx * ny * nz;
-    int64_t b = ns * 2;                                   // pix
+    const int64_t id = nphase_cap > kNarrowPhases ? 4 : 2;  // phase-id width
+    int64_t b = ns * id;                                  // pix
     b += (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ) * kSB * (int64_t)sizeof(PermT);  // lane order
-    b += 7 * 3 * ns * 8 + 6 * ns * 8 + ns * 2;            // u g b h h_alt z Ah, M^-1, homogeneous phase
+    b += 7 * 3 * ns * 8 + 6 * ns * 8 + ns * id;           // u g b h h_alt z Ah, M^-1, homogeneous phase
     b += (int64_t)nphase_cap * ((36 + 576 + kStenHalf + 3 + 6) * 8 + (576 + kStenHalf + 6) * 4);
     // partials (the larger of the streaming and the stencil reductions), state
     const int64_t tiles = (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ);
@@ -327,7 +328,7 @@ int e3x_server_submit(e3x_server *sv, const e3x_request *rq, int64_t *id)
     Server &S = SV(sv);
     if (rq->nx < 2 || rq->ny < 2 || rq->nz < 2) return report_error(E3X_EINVAL, "grid extents must be >= 2");
     if ((int64_t)rq->nx * rq->ny * rq->nz > (int64_t)INT32_MAX) return report_error(E3X_EINVAL, "grid too large");
-    if (rq->nphase < 1 || rq->nphase > 65535) return report_error(E3X_EINVAL, "nphase must be in [1, 65535]");
+    if (rq->nphase < 1 || rq->nphase > kMaxPhases) return report_error(E3X_EINVAL, "nphase must be in [1, 2^31 - 2]");
     if (!rq->pix) return report_error(E3X_EINVAL, "pix is NULL");
     if (rq->flag_m != 0 && rq->nphase > 1 && !rq->orientation)
         return report_error(E3X_EINVAL, "orientation is NULL");

@@ -41,6 +41,11 @@ __global__ __launch_bounds__(kBlock) void k_femat(int nphase, int flag_m, const 
                                                   double *cmod, double *dk, double *sten,
                                                   float *dk32, float *sten32)
 {
+    // Every product and sum rounded on its own, in the order the reference writes them
+    // (f90:2538-2627, 728-803): the tables come out bit for bit as its arithmetic gives
+    // them.  (Fused multiply-adds change the last bits of cmod and dk; the initial energy
+    // at 200^3, a difference of terms ~400 times larger than itself, then moves by ~1e-10.)
+#pragma clang fp contract(off)
     __shared__ double cm[36];
     __shared__ double dks[576];
     const int p = blockIdx.x;
@@ -194,7 +199,8 @@ __global__ __launch_bounds__(kBlock) void k_femat(int nphase, int flag_m, const 
 struct JumpArgs {
     int nx, ny, nz, nrhs;
     int64_t ns;
-    const uint16_t *pix;
+    const void *pix;  // phase ids, uint16 or (wide) uint32
+    int wide;
     const double *dk;
     double e[6 * kMaxRhs];
 };
@@ -257,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *p
                 const int fx = vi == A.nx - 1, fy = vj == A.ny - 1, fz = vk == A.nz - 1;
                 if (!(fx | fy | fz)) continue;
                 const int64_t v = vi + (int64_t)A.nx * (vj + (int64_t)A.ny * vk);
-                const double *D = A.dk + (int64_t)A.pix[v] * 576;
+                const double *D = A.dk + (int64_t)phase_at(A.pix, A.wide, v) * 576;
                 for (int nn = 0; nn < 3; nn++) {
                     double sv = 0.0;
                     for (int m3 = 0; m3 < 3; m3++)
@@ -270,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *p
             // constant of the voxel whose lower corner is m
             const int fx = i == A.nx - 1, fy = j == A.ny - 1, fz = k == A.nz - 1;
             if (fx | fy | fz) {
-                const double *D = A.dk + (int64_t)A.pix[m] * 576;
+                const double *D = A.dk + (int64_t)phase_at(A.pix, A.wide, m) * 576;
                 double c = 0.0;
                 for (int nn = 0; nn < 3; nn++)
                     for (int mm = 0; mm < 8; mm++) {
@@ -352,12 +358,12 @@ __device__ __forceinline__ void precond_block(const double *dk, const int (&ph)[
 }
 
 // Per node: M^-1 (and its FP32 copy), and hphase[m] = the phase of its 8 voxels when they
-// share one, else 0xFFFF; the streaming update takes homogeneous nodes' M^-1 from the
+// share one, else the all-ones id of the handle's width; the streaming update takes homogeneous nodes' M^-1 from the
 // per-phase table (k_precond_table) and reads this array only for the others.  identity:
 // the plain-CG loop's M^-1 = I (z = 1 g0 + 0 g1 + 0 g2 = g exactly for finite g).
 __global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int64_t ns,
-                                                    const uint16_t *pix, const double *dk,
-                                                    double *minv, float *minv32, uint16_t *hphase, int identity)
+                                                    const void *pix, int wide, const double *dk,
+                                                    double *minv, float *minv32, void *hphase, int identity)
 {
     const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (m >= ns) return;
@@ -366,10 +372,11 @@ __global__ __launch_bounds__(kBlock) void k_precond(int nx, int ny, int nz, int6
     bool homog = true;
     for (int a = 0; a < 8; a++) {
         const int64_t v = wrap(i - PX(a), nx) + (int64_t)nx * (wrap(j - PY(a), ny) + (int64_t)ny * wrap(k - PZ(a), nz));
-        ph[a] = pix[v];
+        ph[a] = phase_at(pix, wide, v);
         homog &= ph[a] == ph[0];
     }
-    hphase[m] = homog ? (uint16_t)ph[0] : (uint16_t)0xFFFF;
+    if (wide) static_cast<uint32_t *>(hphase)[m] = homog ? (uint32_t)ph[0] : 0xFFFFFFFFu;
+    else static_cast<uint16_t *>(hphase)[m] = homog ? (uint16_t)ph[0] : (uint16_t)0xFFFF;
     double o[6];
     if (identity) {
         for (int q = 0; q < 6; q++) o[q] = q < 3 ? 1.0 : 0.0;
@@ -406,7 +413,7 @@ JumpArgs jump_args(const Solver &s)
 {
     JumpArgs A{};
     A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.nrhs = s.nrhs; A.ns = s.ns;
-    A.pix = s.pix; A.dk = s.dk;
+    A.pix = s.pix; A.wide = s.wide; A.dk = s.dk;
     for (int q = 0; q < 6 * kMaxRhs; q++) A.e[q] = s.strain[q];
     return A;
 }
@@ -444,7 +451,8 @@ hipError_t launch_precond(Solver &s, bool want32)
 {
     const int identity = s.pcg_variant == 2 ? 1 : 0;
     hipLaunchKernelGGL(k_precond, dim3((unsigned)vector_blocks(s)), dim3(kBlock), 0, s.stream, s.nx, s.ny,
-                       s.nz, s.ns, s.pix, s.dk, s.minv, want32 ? s.minv32 : nullptr, s.hphase, identity);
+                       s.nz, s.ns, (const void *)s.pix, s.wide ? 1 : 0, (const double *)s.dk, s.minv,
+                       want32 ? s.minv32 : nullptr, s.hphase, identity);
     hipLaunchKernelGGL(k_precond_table, dim3((unsigned)((s.nphase + 63) / 64)), dim3(64), 0, s.stream, s.nphase,
                        (const double *)s.dk, s.minv_hom, want32 ? s.minv_hom32 : nullptr, identity);
     s.precond_identity = s.pcg_variant == 2;

@@ -52,7 +52,7 @@ struct StencilArgs {
     FastDiv d_ntile, d_ntx, d_nty;  // by the tiles of one right-hand side, tiles along x, along y
     int wrap1;                      // every extent >= 8: a halo index wraps at most once
     int64_t ns;
-    const uint16_t *pix;
+    const void *pix;       // phase ids: uint16, or uint32 on a wide handle (the kernel's PhT)
     const PermT *perm;     // lane order per tile (Solver::perm)
     const T *dk, *sten;
     const T *x;            // [nrhs][ns][3]
@@ -124,8 +124,8 @@ __device__ __forceinline__ ctab<T> *as_table(const T *p) { return (ctab<T> *)p; 
 // nodes) is issued at the top, before the block learns from the solver state whether
 // its right-hand side still runs, so that the state's latency and the loads' overlap;
 // a stopped right-hand side drops what it loaded and leaves.
-template <typename T, int MODE>
-__device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint16_t *__restrict__ pix,
+template <typename T, int MODE, typename PhT>
+__device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT *__restrict__ pix,
                                              const T *__restrict__ dk, const T *__restrict__ sten,
                                              const T *__restrict__ x, T *__restrict__ y,
                                              const double *__restrict__ bvec, float *__restrict__ y32,
@@ -186,7 +186,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
     for (int k = 0; k < kVoxPer; k++) {
         const int q = min(tid + k * kSB, VN - 1);
         const HPos p = {q % VX, (q / VX) % VY, q / (VX * VY)};
-        pvl[k] = pix[gpos(p)];
+        pvl[k] = (int)pix[gpos(p)];
     }
     // this lane's node: t in the tile's wave-cube numbering (homogeneous nodes by phase,
     // then mixed ones by their phases; k_tile_order)
@@ -431,43 +431,84 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const uint
 
 // At least WPE waves per SIMD: six for the FP64 body (80 VGPRs; forced into 64 VGPRs
 // for eight waves it hoists fewer loads and ran 7 % slower), eight for the FP32 body.
-template <typename T, int MODE, int WPE>
+// PhT: uint16_t phase ids, or uint32_t on a wide handle.
+template <typename T, int MODE, int WPE, typename PhT>
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_stencil(StencilArgs<T> A)
 {
-    stencil_body<T, MODE>(A, A.pix, A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold, A.hnew, A.u);
+    stencil_body<T, MODE, PhT>(A, static_cast<const PhT *>(A.pix), A.dk, A.sten, A.x, A.y, A.b, A.y32, A.z, A.hold,
+                               A.hnew, A.u);
 }
 
 // Lane order of each stencil tile (Solver::perm), from the phases alone: one block per
 // tile, thread t = node t of the tile's wave-cube numbering.  Key: homogeneous nodes
 // first, by phase; then mixed nodes by (smallest, largest, octant-0 phase); nodes
-// outside the grid last; ties by t.  A bitonic sort of the kSB keys in LDS.
-__global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int ntx, int nty, const uint16_t *pix,
+// outside the grid last; ties by t.  A bitonic sort of the kSB keys in LDS.  The phases
+// enter the key as their ranks among the tile's voxel-halo phases (a sorted copy in LDS,
+// at most kSortN of them): the same order as the phases themselves, in 9 bits whatever
+// their width.
+constexpr int kSortN = 512;
+static_assert(VN <= kSortN && kSortN % kSB == 0, "voxel halo must fit the rank table");
+
+template <typename PhT>
+__global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int ntx, int nty, const PhT *pix,
                                                   PermT *perm)
 {
-    __shared__ int ps[VN];
+    __shared__ unsigned ps[VN];
+    __shared__ unsigned srt[kSortN];
     __shared__ unsigned long long key[kSB];
     const int bid = blockIdx.x, t = threadIdx.x;
     const int tx = bid % ntx, ty = (bid / ntx) % nty, tz = bid / (ntx * nty);
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ, nxy = nx * ny;
-    for (int q = t; q < VN; q += kSB)
-        ps[q] = pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
-                    nxy * pmod(oz + q / (VX * VY) - 1, nz)];
+    for (int q = t; q < kSortN; q += kSB) {
+        unsigned v = 0xFFFFFFFFu;
+        if (q < VN) {
+            v = (unsigned)pix[pmod(ox + q % VX - 1, nx) + nx * pmod(oy + (q / VX) % VY - 1, ny) +
+                              nxy * pmod(oz + q / (VX * VY) - 1, nz)];
+            ps[q] = v;
+        }
+        srt[q] = v;
+    }
     __syncthreads();
+    for (int k = 2; k <= kSortN; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < kSortN; i += kSB) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const unsigned a = srt[i], b = srt[p];
+                    if (((i & k) == 0) == (a > b)) {
+                        srt[i] = b;
+                        srt[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // rank of a phase: the position of its first copy in srt
+    auto phase_rank = [&](unsigned v) -> unsigned {
+        int lo = 0, hi = kSortN;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (srt[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        return (unsigned)lo;
+    };
     const int tw = t >> 6, tl = t & 63;
     const int lx = wave_x(tw) + (tl & 3), ly = wave_y(tw) + ((tl >> 2) & 3), lz = wave_z(tw) + (tl >> 4);
-    const int *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
-    unsigned pmin = 0xFFFFu, pmax = 0u;
-    const unsigned p0 = (unsigned)pc[0];
+    const unsigned *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
+    unsigned pmin = 0xFFFFFFFFu, pmax = 0u;
+    const unsigned p0 = pc[0];
 #pragma unroll
     for (int a = 0; a < 8; a++) {
-        const unsigned pa = (unsigned)pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
+        const unsigned pa = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
         pmin = min(pmin, pa);
         pmax = max(pmax, pa);
     }
     const bool active = ox + lx < nx && oy + ly < ny && oz + lz < nz;
     const unsigned long long cls = !active ? 2ull : (pmin == pmax ? 0ull : 1ull);
-    key[t] = (cls << 62) | ((unsigned long long)pmin << 44) | ((unsigned long long)pmax << 28) |
-             ((unsigned long long)p0 << 12) | (unsigned long long)t;
+    key[t] = (cls << 62) | ((unsigned long long)phase_rank(pmin) << 44) |
+             ((unsigned long long)phase_rank(pmax) << 28) | ((unsigned long long)phase_rank(p0) << 12) |
+             (unsigned long long)t;
     __syncthreads();
     for (int k = 2; k <= kSB; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -565,8 +606,9 @@ template <typename T, int MODE>
 void dispatch_stencil(const Solver &s, const StencilArgs<T> &A, int rows = -1)
 {
     const dim3 grid((unsigned)stencil_blocks(s), (unsigned)(rows < 0 ? s.nrhs : rows)), block(kSB);
-    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_stencil<T, MODE, 8>), grid, block, 0, s.stream, A);
-    else hipLaunchKernelGGL((k_stencil<T, MODE, 6>), grid, block, 0, s.stream, A);
+    constexpr int W = sizeof(T) == 4 ? 8 : 6;
+    if (s.wide) hipLaunchKernelGGL((k_stencil<T, MODE, W, uint32_t>), grid, block, 0, s.stream, A);
+    else hipLaunchKernelGGL((k_stencil<T, MODE, W, uint16_t>), grid, block, 0, s.stream, A);
 }
 
 }  // namespace
@@ -670,8 +712,14 @@ hipError_t launch_apply_only(Solver &s, bool mixed, int usteps)
 
 hipError_t launch_tile_order(Solver &s)
 {
-    hipLaunchKernelGGL(k_tile_order, dim3((unsigned)stencil_blocks(s)), dim3(kSB), 0, s.stream, s.nx, s.ny, s.nz,
-                       (s.nx + TX - 1) / TX, (s.ny + TY - 1) / TY, (const uint16_t *)s.pix, s.perm);
+    const dim3 g((unsigned)stencil_blocks(s)), b(kSB);
+    const int ntx = (s.nx + TX - 1) / TX, nty = (s.ny + TY - 1) / TY;
+    if (s.wide)
+        hipLaunchKernelGGL(k_tile_order<uint32_t>, g, b, 0, s.stream, s.nx, s.ny, s.nz, ntx, nty,
+                           (const uint32_t *)s.pix, s.perm);
+    else
+        hipLaunchKernelGGL(k_tile_order<uint16_t>, g, b, 0, s.stream, s.nx, s.ny, s.nz, ntx, nty,
+                           (const uint16_t *)s.pix, s.perm);
     return hipGetLastError();
 }
 

@@ -101,8 +101,10 @@ int e3x_device_count(int *count);
  * Create a solver for an nx*ny*nz periodic grid with nphase phases (grains plus
  * the pore phase) and nrhs simultaneous right-hand sides (1..6; 6 = the unit
  * load cases of the effective stiffness).  Replaces the allocations of
- * program elas3d (f90:242-256).  nphase <= 65535; every extent >= 2 (E3X_EINVAL
- * otherwise: a one-node-thick periodic direction is not supported).
+ * program elas3d (f90:242-256).  nphase <= 2^31 - 2 (phase ids are held in 16 bits up to
+ * 65,535 phases and in 32 bits above, as the reference's anisotropy-sweep inputs with
+ * 81,488 grains need); every extent >= 2 (E3X_EINVAL otherwise: a one-node-thick
+ * periodic direction is not supported).
  */
 int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_solver **out);
 

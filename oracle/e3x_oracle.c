@@ -308,26 +308,30 @@ void oc_dk(int nphase, const double *cmod, double *dk)
     }
 }
 
-/* One boundary voxel of f90:815-1100: b += K_e delta, C += 1/2 delta^T K_e delta. */
+/* One boundary voxel of f90:815-1100: b += K_e delta, and the voxel's constant
+ * 1/2 delta^T K_e delta added to the running chunk sum *C (the constant of the voxel first
+ * summed on its own: its terms in the reference's order, f90:845-846). */
 static void rhs_voxel(int nx, int ny, int nz, int i, int j, int k, const int32_t *pix,
                       const double *dk, const double delta[8][3], double *b, double *C)
 {
     const int64_t ns = (int64_t)nx * ny * nz;
     int64_t m = (int64_t)nx * ny * k + (int64_t)nx * j + i;
     int p = pix[m] - 1;
+    double cv = 0.0;
     for (int nn = 0; nn < 3; nn++)
         for (int mm = 0; mm < 8; mm++) {
             double sumval = 0.0;
             for (int m3 = 0; m3 < 3; m3++)
                 for (int m8 = 0; m8 < 8; m8++) {
                     sumval = sumval + delta[m8][m3] * dk[DK(p, m8, m3, mm, nn)];
-                    *C = *C + 0.5 * delta[m8][m3] * dk[DK(p, m8, m3, mm, nn)] * delta[mm][nn];
+                    cv = cv + 0.5 * delta[m8][m3] * dk[DK(p, m8, m3, mm, nn)] * delta[mm][nn];
                 }
             int i1 = wrapi(i + POS[mm][0], nx), j1 = wrapi(j + POS[mm][1], ny),
                 k1 = wrapi(k + POS[mm][2], nz);
             int64_t ind = (int64_t)nx * ny * k1 + (int64_t)nx * j1 + i1;
             b[nn * ns + ind] += sumval;
         }
+    *C = *C + cv;
 }
 
 /* Jump of a voxel on the upper x/y/z boundaries (f90:823-1080): local node a gets the
@@ -360,28 +364,55 @@ void oc_femat_rhs(int nx, int ny, int nz, const int32_t *pix, const double *dk,
     init_tables();
     const int64_t ns = (int64_t)nx * ny * nz;
     memset(b, 0, sizeof(double) * 3 * (size_t)ns);
+    /* C: the reference sums every term into one running total per OpenMP thread
+     * (reduction(+:C), f90:833); over the ~1e5 voxels x 576 terms of a 200^3 grid such a
+     * running total drifts by ~1e-12 of C, which the energy (C and b.u cancel to ~1/200 of
+     * C) turns into ~2e-10.  Here each voxel's constant is summed on its own, those of a
+     * grid line in a running sum, and the lines' sums in a second running sum: within
+     * ~1e-16 of the exact total (pinned against an exact sum). */
     double c = 0.0, delta[8][3];
     /* face x = nx (f90:821-854) */
     jump_delta(1, 0, 0, nx, ny, nz, strain6, delta);
-    for (int j = 0; j < ny - 1; j++)
-        for (int k = 0; k < nz - 1; k++) rhs_voxel(nx, ny, nz, nx - 1, j, k, pix, dk, delta, b, &c);
+    for (int j = 0; j < ny - 1; j++) {
+        double cl = 0.0;
+        for (int k = 0; k < nz - 1; k++) rhs_voxel(nx, ny, nz, nx - 1, j, k, pix, dk, delta, b, &cl);
+        c = c + cl;
+    }
     /* face y = ny (f90:856-886) */
     jump_delta(0, 1, 0, nx, ny, nz, strain6, delta);
-    for (int i = 0; i < nx - 1; i++)
-        for (int k = 0; k < nz - 1; k++) rhs_voxel(nx, ny, nz, i, ny - 1, k, pix, dk, delta, b, &c);
+    for (int i = 0; i < nx - 1; i++) {
+        double cl = 0.0;
+        for (int k = 0; k < nz - 1; k++) rhs_voxel(nx, ny, nz, i, ny - 1, k, pix, dk, delta, b, &cl);
+        c = c + cl;
+    }
     /* face z = nz (f90:888-918) */
     jump_delta(0, 0, 1, nx, ny, nz, strain6, delta);
-    for (int i = 0; i < nx - 1; i++)
-        for (int j = 0; j < ny - 1; j++) rhs_voxel(nx, ny, nz, i, j, nz - 1, pix, dk, delta, b, &c);
+    for (int i = 0; i < nx - 1; i++) {
+        double cl = 0.0;
+        for (int j = 0; j < ny - 1; j++) rhs_voxel(nx, ny, nz, i, j, nz - 1, pix, dk, delta, b, &cl);
+        c = c + cl;
+    }
     /* edge x=nx, y=ny (f90:920-959) */
     jump_delta(1, 1, 0, nx, ny, nz, strain6, delta);
-    for (int k = 0; k < nz - 1; k++) rhs_voxel(nx, ny, nz, nx - 1, ny - 1, k, pix, dk, delta, b, &c);
+    {
+        double cl = 0.0;
+        for (int k = 0; k < nz - 1; k++) rhs_voxel(nx, ny, nz, nx - 1, ny - 1, k, pix, dk, delta, b, &cl);
+        c = c + cl;
+    }
     /* edge x=nx, z=nz (f90:961-999) */
     jump_delta(1, 0, 1, nx, ny, nz, strain6, delta);
-    for (int j = 0; j < ny - 1; j++) rhs_voxel(nx, ny, nz, nx - 1, j, nz - 1, pix, dk, delta, b, &c);
+    {
+        double cl = 0.0;
+        for (int j = 0; j < ny - 1; j++) rhs_voxel(nx, ny, nz, nx - 1, j, nz - 1, pix, dk, delta, b, &cl);
+        c = c + cl;
+    }
     /* edge y=ny, z=nz (f90:1001-1039) */
     jump_delta(0, 1, 1, nx, ny, nz, strain6, delta);
-    for (int i = 0; i < nx - 1; i++) rhs_voxel(nx, ny, nz, i, ny - 1, nz - 1, pix, dk, delta, b, &c);
+    {
+        double cl = 0.0;
+        for (int i = 0; i < nx - 1; i++) rhs_voxel(nx, ny, nz, i, ny - 1, nz - 1, pix, dk, delta, b, &cl);
+        c = c + cl;
+    }
     /* corner (f90:1041-1094) */
     jump_delta(1, 1, 1, nx, ny, nz, strain6, delta);
     rhs_voxel(nx, ny, nz, nx - 1, ny - 1, nz - 1, pix, dk, delta, b, &c);

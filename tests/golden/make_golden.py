@@ -68,12 +68,56 @@ def case_void():
                 history=r["history"], u=r["u"])
 
 
+# Nodes at which the 200^3 fixtures keep the displacement: every CUBE200_STRIDE-th node
+# (2,998 of 8,000,000, spread over every z plane).
+CUBE200_STRIDE = 2671
+
+
+def case_cube200(zener):
+    """The 200^3 workload at Zener ratio `zener`: 500 periodic Voronoi grains from seeded
+    centres (the generator of elas3d-xtal_amd, seed 2001), uniform SO(3) orientations,
+    C12 = 147.3, C44 = 124.7 GPa, C11 = C12 + 2 C44 / A, 1 % lognormal spherical pores,
+    exx = float32(1e-3), the reference's PCG controls.  Kept: a hash of the generated
+    microstructure (the test regenerates it), the solve's scalars and residual history,
+    and the displacement at a fixed sample of nodes."""
+    import hashlib
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "elas3d-xtal_amd"))
+    from elas3dxtal import configs
+
+    w = configs.build("cube200")
+    c11, c12, c44 = configs.materials("cube200", zener=zener)
+    shape, pix, rod = w["shape"], w["pix"], w["rod"]
+    e6 = np.array([oc.AML_F32, 0, 0, 0, 0, 0])
+    r = oc.solve(shape, w["nphase"], pix, rod, c11, c12, c44, e6)
+    ns = int(np.prod(shape))
+    idx = np.arange(0, ns, CUBE200_STRIDE)
+    u = r["u"].reshape(3, ns)[:, idx]
+    return dict(shape=np.array(shape), nphase=w["nphase"], zener=zener, c=np.array([c11, c12, c44]),
+                pix_sha256=hashlib.sha256(np.ascontiguousarray(pix, np.int32).tobytes()).hexdigest(),
+                rod_sha256=hashlib.sha256(np.ascontiguousarray(rod, np.float64).tobytes()).hexdigest(),
+                strain6=e6, lstep=r["lstep"], status=r["status"], reset_count=r["reset_count"],
+                utot0=r["utot0"], gg0=r["gg0"], utot=r["utot"], gg=r["gg"], stress6=r["stress6"],
+                strain_avg6=r["strain6"], history=r["history"], u_index=idx, u_sample=u)
+
+
 CASES = {"poly_10x9x8": case_poly, "reset_6x6x6": case_reset, "void_12x12x10": case_void}
+CUBE200 = {f"cube200_A{a:g}".replace(".", "p"): a for a in (0.5, 1.0, 2.0, 5.0)}
 
 if __name__ == "__main__":
-    oc.lib.oc_set_threads(1)   # one thread: the oracle's sums in one fixed order
-    for name, fn in CASES.items():
+    # usage: make_golden.py            the small cases (one thread: one fixed order)
+    #        make_golden.py cube200    the four 200^3 cases (the oracle's reductions are
+    #                                  fixed-order chunked sums, the same bits for any
+    #                                  thread count; all threads, ~2-4 min per case)
+    if len(sys.argv) > 1 and sys.argv[1] == "cube200":
+        todo = {name: (lambda a=a: case_cube200(a)) for name, a in CUBE200.items()}
+        if len(sys.argv) > 2:
+            todo = {k: v for k, v in todo.items() if k in sys.argv[2:]}
+    else:
+        oc.lib.oc_set_threads(1)
+        todo = CASES
+    for name, fn in todo.items():
         d = fn()
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **{k: np.asarray(v) for k, v in d.items()})
-        print(name, "lstep", int(d["lstep"]), "status", int(d["status"]), os.path.getsize(path), "bytes")
+        print(name, "lstep", int(d["lstep"]), "status", int(d["status"]), os.path.getsize(path), "bytes", flush=True)

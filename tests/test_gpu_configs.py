@@ -1,5 +1,7 @@
 """The benchmark workloads at their full sizes on the GPU.
 
+- 801^3 (the reference's production grid, 81,488 grains: 32-bit phase ids), FP64, checked
+  by invariants.
 - 50^3, 20 grains, 2 % single-voxel pores, moduli in GPa (C11 = 246.5, C12 = 147.3,
   C44 = 124.7), exx = 1e-3, stopped when the sum of squared residuals falls below 1e-8
   times the voxel count (NIST ELAS3D's absolute rule): against the oracle, which runs
@@ -68,6 +70,47 @@ def test_cube50_nist_rule_gpa_matches_oracle():
     assert _rel(res.stress6[0], ro["stress6"]) < 1e-10
     assert _rel(res.strain6[0], ro["strain6"]) < 1e-12
     assert _rel(res.history[0], ro["history"]) < 1e-8
+
+
+def test_cube801_production_grid():
+    """The reference's production grid, as the anisotropy-sweep program for Zener ratio 2
+    sets it up: md = 801 nodes per axis (applications/sec3_2/A_2_0/elas3dxtal_pcg.f90:52),
+    514 M nodes; 81,488 grains (A_2_0:59) plus the pore phase, so the handle holds 32-bit
+    phase ids; C11, C12, C44 = 216.52, 126.70, 89.82 GPa (A_2_0:67-69); ezz =
+    float32(1e-3) (A_2_0:83-87); that program's PCG loop (no divergence reset, stop test
+    before the direction update: E3X_PCG_SWEEP) with tol = float32(1e-8).  The
+    microstructure is the seeded generator's (periodic Voronoi, uniform SO(3)
+    orientations, 1 % lognormal spherical pores), not the sec3_2 MATLAB input.  One FP64
+    solve, about 115 GB of device memory.  Too large for the oracle, so it is checked by
+    what the mathematics fixes: the solve converges and the energy falls; the averaged
+    strain is the applied strain (for any periodic u, f90:2252-2391); and Hill's condition
+    holds at convergence, utot / ns = 1/2 <sigma> . <eps>."""
+    import time
+
+    from elas3dxtal import io
+    shape, ng = (801, 801, 801), 81488
+    ns = 801 ** 3
+    t0 = time.perf_counter()
+    pix, rod, _, frac = io.generate(shape, ng, 8001, pore_mode=io.PORES_SPHERES, pore_fraction=0.01,
+                                    pore_median=3.0, pore_sigma=0.4)
+    t_gen = time.perf_counter() - t0
+    c11, c12, c44 = 216.52e9, 126.70e9, 89.82e9
+    ezz = float(np.float32(1e-3))
+    e6 = [0.0, 0.0, ezz, 0.0, 0.0, 0.0]
+    t0 = time.perf_counter()
+    res = solve(pix, rod, shape, ng + 1, c11, c12, c44, e6, variant=L.PCG_SWEEP)
+    wall = time.perf_counter() - t0
+    del pix
+    K = int(res.lstep[0])
+    pcg_s = res.timing["pcg_ms"] * 1e-3
+    print(f"801^3, {ng} grains, pores {frac:.4f}: generation {t_gen:.1f} s; solve {wall:.1f} s (femat to "
+          f"averages, upload included), PCG {K} iterations in {pcg_s:.2f} s = {3.0 * ns * K / pcg_s:.3e} "
+          f"DOF-updates/s; utot0 {res.utot0[0]:.6e} utot {res.utot[0]:.6e}")
+    assert res.exit[0] == L.PCG_CONVERGED and 1 < K < 10000
+    assert res.utot[0] < res.utot0[0]
+    assert _rel(res.strain6[0], e6) < 1e-9
+    hill = 0.5 * float(np.dot(res.stress6[0], res.strain6[0]))
+    assert abs(res.utot[0] / ns - hill) < 1e-6 * abs(res.utot[0] / ns)
 
 
 def test_cube300_mixed_against_fp64():

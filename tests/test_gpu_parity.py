@@ -47,8 +47,9 @@ def test_femat_tables_rhs_constant(shape, ngrain):
         b = s.get(L.FIELD_B)
         prob = s.assig()
     cm_o, dk_o = oracle_tables(nphase, rod)
-    assert _rel(cm, cm_o) < 1e-13
-    assert _rel(dk, dk_o) < 1e-13
+    # the same operations in the same order, each rounded on its own: the same bits
+    assert np.array_equal(cm, cm_o)
+    assert np.array_equal(dk, dk_o)
     b_o, c_o = oc.femat_rhs(shape, pix, dk_o, e6)
     assert _rel(b, b_o) < 1e-12
     assert abs(cval[0] - c_o) < 1e-12 * abs(c_o)
@@ -625,6 +626,68 @@ def test_effective_stiffness_single_crystal():
     c[[3, 4, 5], [3, 4, 5]] = C44
     assert np.all(ex == L.PCG_HAH_BREAK)      # the affine guess is the solution
     assert _rel(ce, c) < 1e-12
+
+
+def test_more_than_65535_phases_matches_oracle():
+    """A 48^3 grid with 70,000 grains (the reference's anisotropy-sweep inputs have 81,488,
+    applications/sec3_2/A_2_0/elas3dxtal_pcg.f90:59) plus the pore phase: the handle holds
+    32-bit phase ids.  Labels above 65,535 occur; the solve matches the oracle as the
+    16-bit path does."""
+    from elas3dxtal import io
+    shape, ng = (48, 48, 48), 70000
+    pix, rod, _, _ = io.generate(shape, ng, 4801, pore_mode=io.PORES_VOXELS, pore_fraction=0.02)
+    assert pix.max() == ng + 1 and np.count_nonzero(pix > 65535) > 1000
+    e6 = [oc.AML_F32, 0, 0, 0, 0, 0]
+    ro = oc.solve(shape, ng + 1, pix, rod, C11, C12, C44, e6)
+    res, s = solve(pix, rod, shape, ng + 1, C11, C12, C44, e6, keep=True)
+    try:
+        u = s.get(L.FIELD_U)
+        prob = s.assig()
+        _, vm = s.stress_fullfield()
+    finally:
+        s.close()
+    K = int(res.lstep[0])
+    print(f"70,001 phases: lstep GPU {K}, oracle {ro['lstep']}")
+    assert res.exit[0] == L.PCG_CONVERGED
+    assert abs(K - ro["lstep"]) <= max(1, int(0.01 * ro["lstep"]))
+    if K != ro["lstep"]:
+        ro = oc.solve(shape, ng + 1, pix, rod, C11, C12, C44, e6, tol=0.0, ldemb=K)
+    assert np.array_equal(prob, oc.assig(pix, ng + 1))
+    assert abs(res.utot0[0] - ro["utot0"]) < 1e-10 * abs(ro["utot0"])
+    assert abs(res.utot[0] - ro["utot"]) < 1e-10 * abs(ro["utot"])
+    assert _rel(u, ro["u"]) < 1e-10
+    assert _rel(res.stress6[0], ro["stress6"]) < 1e-10
+    assert _rel(res.history[0], ro["history"]) < 1e-8
+    cm_o = oc.cmod(ng + 1, oc.rotate_stiffness(rod, C11, C12, C44), [0.0, 0.0])
+    _, vm_o = oc.fullfield(shape, pix, cm_o, e6, ro["u"])
+    assert _rel(vm, vm_o) < 1e-9
+
+
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_wide_phase_ids_give_the_same_bits(precision):
+    """The same microstructure on a handle with 16-bit phase ids and on one with 32-bit
+    ids (nphase above 65,535, the extra grains unused): the lane order of the stencil
+    tiles ranks phases within each tile, so both take the same order and every sum the
+    same order of operations: identical results."""
+    shape = (20, 18, 16)
+    pix, rod = voronoi_case(shape, 9, seed=4802, pore_frac=0.03)
+    pix = np.where(pix == 10, 70001, pix).astype(np.int32)   # the pore phase as the last of 70,001
+    rng = np.random.default_rng(4803)
+    rod_wide = np.concatenate([rod, ind.random_rodrigues(rng, 70000 - 9)])
+    pix_narrow = np.where(pix == 70001, 10, pix).astype(np.int32)
+    e6 = [oc.AML_F32, 0, 0, 0, 0.3e-3, 0]
+    kw = dict(precision=precision, replace_every=7)
+    narrow, s1 = solve(pix_narrow, rod, shape, 10, C11, C12, C44, e6, keep=True, **kw)
+    wide, s2 = solve(pix, rod_wide, shape, 70001, C11, C12, C44, e6, keep=True, **kw)
+    try:
+        u1, u2 = s1.get(L.FIELD_U), s2.get(L.FIELD_U)
+    finally:
+        s1.close()
+        s2.close()
+    assert narrow.lstep[0] == wide.lstep[0] and narrow.exit[0] == wide.exit[0] == L.PCG_CONVERGED
+    assert np.array_equal(u1, u2)
+    assert narrow.utot[0] == wide.utot[0]
+    assert np.array_equal(narrow.stress6[0], wide.stress6[0])
 
 
 def test_bad_phase_label_and_call_order():

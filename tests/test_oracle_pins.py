@@ -168,6 +168,21 @@ def test_rhs_and_constant_equal_jump_assembly(shape):
     assert abs(cval - c_ref) < 1e-13 * abs(c_ref)
 
 
+def test_jump_constant_to_rounding_on_a_large_grid():
+    """The constant C summed over the ~5e4 boundary voxels of a 128^3 grid agrees with the
+    exact sum of the voxel terms (computed independently, grouped by wrap pattern) to a
+    few ulps: summed as one running total, as the reference's reduction does, it drifts by
+    ~1e-12 here, which the energy amplifies ~200-fold through the cancellation of C
+    against b.u."""
+    shape = (128, 128, 128)
+    pix, _, _, dk = _random_case(shape, 40, seed=21)
+    e6 = [1e-3, 0.0, 0.0, 0.0, 0.0, 0.0]
+    _, cval = oc.femat_rhs(shape, pix, dk, e6)
+    exact = ind.jump_constant_exact(shape, pix, ind.dk_to_kmats(dk), e6)
+    print(f"C oracle {cval!r}, exact {exact!r}, rel {abs(cval - exact) / abs(exact):.2e}")
+    assert abs(cval - exact) < 1e-15 * abs(exact)
+
+
 @pytest.mark.parametrize("shape", [(3, 3, 3), (4, 5, 3)])
 def test_energy_equals_total_field_element_energy(shape):
     pix, _, _, dk = _random_case(shape, 4, seed=30)
