@@ -84,7 +84,7 @@ int ensure_mixed_buffers(Solver &s)
     E3X_HIP(dalloc(s, &s.h32_alt, nv));
     E3X_HIP(dalloc(s, &s.z32, nv));
     E3X_HIP(dalloc(s, &s.ah32, nv));
-    E3X_HIP(dalloc(s, &s.minv32, 6 * s.ns));
+    E3X_HIP(dalloc(s, &s.minv32, 6 * s.ns * s.member_count()));
     // the buffers a fused apply reads before any PCG has written them (z and the
     // direction of iteration 1, e.g. under e3x_time_apply) start as zeros, not as stale
     // bits that lambda = 0 would turn into NaN
@@ -127,6 +127,28 @@ int report_error(int code, const char *msg)
     return fail(code, msg ? msg : "");
 }
 
+int solver_set_member_layout(Solver &s, const int *nphase)
+{
+    if (s.run.on) return fail(E3X_ESTATE, "a PCG is in progress on this handle");
+    if (!s.members) return fail(E3X_ESTATE, "not a batch handle");
+    int64_t tot = 0;
+    for (int r = 0; r < s.members; r++) {
+        if (nphase[r] < 1) return fail(E3X_EINVAL, "every member needs at least one phase");
+        tot += nphase[r];
+    }
+    if (tot > s.nphase_cap) return fail(E3X_EINVAL, "the members' phases exceed the tables of this handle");
+    int base = 0;
+    for (int r = 0; r < s.members; r++) {
+        s.phase_base[r] = base;
+        s.member_nphase[r] = nphase[r];
+        base += nphase[r];
+    }
+    s.nphase = base;
+    s.member_set = 0;
+    s.have_pix = s.have_femat = s.have_precond = s.have_precond32 = false;
+    return E3X_OK;
+}
+
 }  // namespace e3x
 
 extern "C" {
@@ -149,16 +171,31 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
 {
     if (!out) return fail(E3X_EINVAL, "out is NULL");
     *out = nullptr;
+    Solver *s = nullptr;
+    if (int rc = solver_create(nx, ny, nz, nphase, nrhs, 0, device, &s)) return rc;
+    *out = reinterpret_cast<e3x_solver *>(s);
+    return E3X_OK;
+}
+
+}  // extern "C"
+
+namespace e3x {
+
+int solver_create(int nx, int ny, int nz, int nphase, int nrhs, int members, int device, Solver **out)
+{
+    *out = nullptr;
     if (nx < 2 || ny < 2 || nz < 2) return fail(E3X_EINVAL, "grid extents must be >= 2");
     if ((int64_t)nx * ny * nz > (int64_t)INT32_MAX) return fail(E3X_EINVAL, "grid exceeds 2^31 nodes");
     if (nphase < 1 || nphase > kMaxPhases) return fail(E3X_EINVAL, "nphase must be in [1, 2^31 - 2]");
     if (nrhs < 1 || nrhs > kMaxRhs) return fail(E3X_EINVAL, "nrhs must be in [1, 6]");
+    if (members != 0 && members != nrhs) return fail(E3X_EINVAL, "a batch has one right-hand side per member");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(E3X_EDEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return fail(E3X_EINVAL, "device index out of range");
     Solver *s = new (std::nothrow) Solver();
     if (!s) return fail(E3X_ENOMEM, "host allocation failed");
     s->nx = nx; s->ny = ny; s->nz = nz; s->nphase = nphase; s->nphase_cap = nphase; s->nrhs = nrhs;
+    s->members = members;
     s->wide = nphase > kNarrowPhases;
     s->device = device;
     s->ns = (int64_t)nx * ny * nz;
@@ -185,8 +222,9 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->sten32, (int64_t)nphase * kStenHalf));
     CK(dalloc(*s, &s->orient, (int64_t)3 * std::max(nphase - 1, 1)));
     const int64_t id_bytes = s->wide ? 4 : 2;  // phase-id width
-    CK(dalloc(*s, reinterpret_cast<uint8_t **>(&s->pix), id_bytes * ns));
-    CK(dalloc(*s, &s->perm, stencil_blocks(*s) * kSB));
+    const int64_t nm = s->member_count();      // microstructures held
+    CK(dalloc(*s, reinterpret_cast<uint8_t **>(&s->pix), id_bytes * ns * nm));
+    CK(dalloc(*s, &s->perm, stencil_blocks(*s) * kSB * nm));
     CK(dalloc(*s, &s->u, nv));
     CK(dalloc(*s, &s->g, nv));
     CK(dalloc(*s, &s->b, nv));
@@ -194,8 +232,8 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(dalloc(*s, &s->h_alt, nv));
     CK(dalloc(*s, &s->z, nv));
     CK(dalloc(*s, &s->ah, nv));
-    CK(dalloc(*s, &s->minv, 6 * ns));
-    CK(dalloc(*s, reinterpret_cast<uint8_t **>(&s->hphase), id_bytes * ns));
+    CK(dalloc(*s, &s->minv, 6 * ns * nm));
+    CK(dalloc(*s, reinterpret_cast<uint8_t **>(&s->hphase), id_bytes * ns * nm));
     CK(dalloc(*s, &s->minv_hom, (int64_t)nphase * 6));
     CK(dalloc(*s, &s->minv_hom32, (int64_t)nphase * 6));
     CK(dalloc(*s, &s->cconst, kMaxRhs));
@@ -216,7 +254,103 @@ int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_sol
     CK(hipMemset(s->z, 0, sizeof(double) * (size_t)nv));
     CK(hipMemset(s->u, 0, sizeof(double) * (size_t)nv));
 #undef CK
+    *out = s;
+    return E3X_OK;
+}
+
+}  // namespace e3x
+
+extern "C" {
+
+int e3x_create_batch(int nx, int ny, int nz, int members, const int *nphase, int device, e3x_solver **out)
+{
+    if (!out) return fail(E3X_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (members < 2 || members > kMaxRhs) return fail(E3X_EINVAL, "members must be in [2, 6]");
+    if (!nphase) return fail(E3X_EINVAL, "nphase is NULL");
+    int64_t tot = 0;
+    for (int r = 0; r < members; r++) {
+        if (nphase[r] < 1) return fail(E3X_EINVAL, "every member needs at least one phase");
+        tot += nphase[r];
+    }
+    if (tot > kMaxPhases) return fail(E3X_EINVAL, "the members' phases exceed 2^31 - 2");
+    Solver *s = nullptr;
+    if (int rc = solver_create(nx, ny, nz, (int)tot, members, members, device, &s)) return rc;
+    if (int rc = solver_set_member_layout(*s, nphase)) {
+        release(*s);
+        delete s;
+        return rc;
+    }
     *out = reinterpret_cast<e3x_solver *>(s);
+    return E3X_OK;
+}
+
+int e3x_set_member(e3x_solver *h, int member, const int32_t *pix, const double *orientation, double c11,
+                   double c12, double c44, const double *phasemod, int flag_m, const double *strain6)
+{
+    if (int rc = check_idle(h)) return rc;
+    Solver &s = S(h);
+    if (!s.members) return fail(E3X_ESTATE, "e3x_set_member needs a batch handle (e3x_create_batch)");
+    if (member < 0 || member >= s.members) return fail(E3X_EINVAL, "member out of range");
+    if (!pix || !strain6) return fail(E3X_EINVAL, "pix or strain6 is NULL");
+    if (flag_m != 0 && flag_m != 1) return fail(E3X_EINVAL, "flag_m must be 0 or 1");
+    const int np = s.member_nphase[member], base = s.phase_base[member];
+    if (flag_m == 0 && np != 2) return fail(E3X_EINVAL, "flag_m = 0 needs nphase = 2");
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t m = 0; m < s.ns; m++) {
+        lo = std::min(lo, pix[m]);
+        hi = std::max(hi, pix[m]);
+    }
+    if (lo < 1 || hi > np)  // f90:2986-2989, against the member's phases
+        return fail(E3X_EPHASE, "phase label in pix out of bounds: min " + std::to_string(lo) + ", max " +
+                                    std::to_string(hi) + ", nphase " + std::to_string(np));
+    E3X_HIP(hipSetDevice(s.device));
+    // the member's ids, shifted to its slice of the shared tables
+    const int64_t n0 = (int64_t)member * s.ns;
+    std::vector<uint16_t> p16;
+    std::vector<uint32_t> p32;
+    if (s.wide) {
+        p32.resize((size_t)s.ns);
+        for (int64_t m = 0; m < s.ns; m++) p32[(size_t)m] = (uint32_t)(pix[m] - 1 + base);
+        E3X_HIP(hipMemcpyAsync(static_cast<uint32_t *>(s.pix) + n0, p32.data(), sizeof(uint32_t) * (size_t)s.ns,
+                               hipMemcpyHostToDevice, s.stream));
+    } else {
+        p16.resize((size_t)s.ns);
+        for (int64_t m = 0; m < s.ns; m++) p16[(size_t)m] = (uint16_t)(pix[m] - 1 + base);
+        E3X_HIP(hipMemcpyAsync(static_cast<uint16_t *>(s.pix) + n0, p16.data(), sizeof(uint16_t) * (size_t)s.ns,
+                               hipMemcpyHostToDevice, s.stream));
+    }
+    // orientations of the member's grains, at its slice (grains = phases less the pore)
+    const double *ori = nullptr;
+    if (orientation && np > 1) {
+        E3X_HIP(hipMemcpyAsync(s.orient + 3 * (int64_t)base, orientation, sizeof(double) * 3 * (size_t)(np - 1),
+                               hipMemcpyHostToDevice, s.stream));
+        ori = s.orient + 3 * (int64_t)base;
+    } else if (flag_m == 1 && np > 1) {
+        return fail(E3X_EINVAL, "orientation is NULL");
+    }
+    E3X_HIP(launch_tile_order(s, member));
+    double pm[4] = {0, 0, 0, 0};
+    if (phasemod) {
+        pm[0] = phasemod[0];
+        pm[1] = phasemod[1];
+        if (flag_m == 0) {
+            pm[2] = phasemod[2];
+            pm[3] = phasemod[3];
+        }
+    }
+    s.flag_m = flag_m;
+    E3X_HIP(launch_femat_range(s, base, np, ori ? ori : s.orient, c11, c12, c44, pm));
+    for (int q = 0; q < 6; q++) s.strain[6 * member + q] = strain6[q];
+    s.member_set |= 1u << member;
+    s.have_precond = s.have_precond32 = false;
+    s.have_pix = s.have_femat = false;
+    if (s.member_set == (1u << s.members) - 1u) {
+        // every member is set: the jump right-hand sides and constants of all (f90:815-1100)
+        E3X_HIP(launch_rhs(s));
+        s.have_pix = s.have_femat = true;
+    }
+    E3X_HIP(hipStreamSynchronize(s.stream));
     return E3X_OK;
 }
 
@@ -243,6 +377,7 @@ int e3x_set_microstructure(e3x_solver *h, const int32_t *pix, const double *orie
 {
     if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
+    if (s.members) return fail(E3X_ESTATE, "e3x_set_microstructure is not for a batch handle (e3x_set_member)");
     if (!pix) return fail(E3X_EINVAL, "pix is NULL");
     int32_t lo = INT32_MAX, hi = INT32_MIN;
     for (int64_t m = 0; m < s.ns; m++) {
@@ -282,6 +417,7 @@ int e3x_assig(e3x_solver *h, double *prob)
 {
     if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
+    if (s.members) return fail(E3X_ESTATE, "e3x_assig is not for a batch handle (e3x_set_member)");
     if (!prob) return fail(E3X_EINVAL, "prob is NULL");
     if (!s.have_pix) return fail(E3X_ESTATE, "e3x_assig before e3x_set_microstructure");
     // f90:2455-2465: counts of each label over ns (exact integers, counted on upload)
@@ -294,6 +430,7 @@ int e3x_femat(e3x_solver *h, double c11, double c12, double c44, const double *p
 {
     if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
+    if (s.members) return fail(E3X_ESTATE, "e3x_femat is not for a batch handle (e3x_set_member)");
     if (!s.have_pix) return fail(E3X_ESTATE, "e3x_femat before e3x_set_microstructure");
     if (flag_m != 0 && flag_m != 1) return fail(E3X_EINVAL, "flag_m must be 0 or 1");
     if (flag_m == 0 && s.nphase != 2) return fail(E3X_EINVAL, "flag_m = 0 needs nphase = 2");
@@ -662,7 +799,7 @@ int e3x_hotspots(e3x_solver *h, int rhs, const double *vm, int nsolid, double si
     if (vm) E3X_HIP(hipMemcpyAsync(dvm, vm, sizeof(double) * (size_t)s.ns, hipMemcpyHostToDevice, s.stream));
     else E3X_HIP(launch_fullfield(s, rhs, nullptr, dvm));
     HotResult res;
-    E3X_HIP(launch_hotspots(s, nsolid, sigma, (float)threshold_mpa, npeaks, res));
+    E3X_HIP(launch_hotspots(s, rhs, nsolid, sigma, (float)threshold_mpa, npeaks, res));
     for (size_t k = 0; k < res.peaks.size(); k++) {
         index[k] = res.peaks[k].m;
         value[k] = (double)res.peaks[k].v;
@@ -681,6 +818,7 @@ int e3x_effective_stiffness(e3x_solver *h, double c11, double c12, double c44, c
 {
     if (int rc = check_idle(h)) return rc;
     Solver &s = S(h);
+    if (s.members) return fail(E3X_ESTATE, "e3x_effective_stiffness is not for a batch handle (e3x_set_member)");
     if (s.nrhs != 6) return fail(E3X_EINVAL, "e3x_effective_stiffness needs a handle with nrhs = 6");
     if (!C_eff) return fail(E3X_EINVAL, "C_eff is NULL");
     if (!(magnitude != 0.0) || !std::isfinite(magnitude)) return fail(E3X_EINVAL, "magnitude must be finite and nonzero");

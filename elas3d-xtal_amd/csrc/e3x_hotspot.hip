@@ -155,12 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_hot_topk(const HotEntry *cand, const
 
 }  // namespace
 
-hipError_t launch_hotspots(Solver &s, int nsolid, double sigma, float thr_mpa, int npeaks, HotResult &res)
+hipError_t launch_hotspots(Solver &s, int rhs, int nsolid, double sigma, float thr_mpa, int npeaks, HotResult &res)
 {
     const int64_t ns = s.ns;
     HotArgs A{};
     A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.ns = ns; A.nsolid = nsolid;
-    A.pix = s.pix;
+    // the phase map of the right-hand side's member (a batch), else the shared one
+    A.pix = (const char *)s.pix + (s.wide ? 4 : 2) * (int64_t)rhs * s.mstride();
     A.wide = s.wide ? 1 : 0;
     A.vm = s.post + 6 * ns;
     A.sm = reinterpret_cast<float *>(s.post);

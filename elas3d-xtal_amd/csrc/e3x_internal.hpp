@@ -79,6 +79,17 @@ struct Solver {
     int nx = 0, ny = 0, nz = 0, nphase = 0, nrhs = 1, device = 0;
     int nphase_cap = 0;  // phases the tables are allocated for (nphase <= nphase_cap)
     int64_t ns = 0;
+    // A batch of separate problems of one grid size (e3x_create_batch; the serving loop's
+    // co-batched requests): with members > 0, right-hand side r is a problem of its own,
+    // with its own microstructure (pix, lane order, hphase and M^-1 at member offset r),
+    // its phases at phase_base[r] .. phase_base[r] + member_nphase[r] - 1 of the shared
+    // tables (pix holds those shifted ids), its materials and its load.  0: the
+    // right-hand sides share one microstructure.
+    int members = 0;
+    int phase_base[kMaxRhs] = {}, member_nphase[kMaxRhs] = {};
+    unsigned member_set = 0;  // members whose microstructure and tables are loaded (bit r)
+    int64_t mstride() const { return members ? ns : 0; }  // nodes between members' per-node arrays
+    int64_t member_count() const { return members ? members : 1; }
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     int flag_m = 1;
@@ -168,7 +179,10 @@ struct Solver {
 hipError_t launch_femat(Solver &s, double c11, double c12, double c44, const double *phasemod4);
 hipError_t launch_rhs(Solver &s);                             // b and cconst
 hipError_t launch_affine(Solver &s);
-hipError_t launch_precond(Solver &s, bool want32);
+hipError_t launch_precond(Solver &s, bool want32);  // every member of a batch
+// the tables of phases base .. base + nphase - 1 (a batch member's; 0, s.nphase otherwise)
+hipError_t launch_femat_range(Solver &s, int base, int nphase, const double *orient, double c11, double c12,
+                              double c44, const double *phasemod4);
 // e3x_stencil.hip
 hipError_t launch_energy(Solver &s);                          // results[0..2*nrhs)
 hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K x (FP64)
@@ -177,7 +191,7 @@ hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K 
 hipError_t launch_apply_hah(Solver &s, bool mixed, int it, int usteps);
 hipError_t launch_apply_only(Solver &s, bool mixed, int usteps);  // fused apply with the current state
 hipError_t launch_resid(Solver &s);                           // g32 = K u + b in FP64
-hipError_t launch_tile_order(Solver &s);                      // perm from pix
+hipError_t launch_tile_order(Solver &s, int member = 0);      // perm from pix (of a batch member)
 // e3x_pcg.hip
 hipError_t launch_hah_finish(Solver &s, int nparts, int it);
 hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
@@ -208,12 +222,18 @@ struct HotResult {
     int64_t found = 0;
     std::vector<HotEntry> peaks;
 };
-hipError_t launch_hotspots(Solver &s, int nsolid, double sigma, float thr_mpa, int npeaks, HotResult &res);
+hipError_t launch_hotspots(Solver &s, int rhs, int nsolid, double sigma, float thr_mpa, int npeaks, HotResult &res);
 
 // e3x_api.hip: reuse a handle for another microstructure with nphase <= nphase_cap
 // phases (the serving loop's handle pool); the microstructure and tables must be
 // loaded again.  Returns an e3x_status.
 int solver_set_nphase(Solver &s, int nphase);
+// e3x_api.hip: a handle for nrhs right-hand sides sharing one microstructure (members = 0)
+// or for `members` separate problems (nrhs = members); an e3x_status
+int solver_create(int nx, int ny, int nz, int nphase, int nrhs, int members, int device, Solver **out);
+// e3x_api.hip: the phase counts of a batch's members (their slices of the tables, in
+// order; the sum at most nphase_cap), before their e3x_set_member; an e3x_status
+int solver_set_member_layout(Solver &s, const int *nphase);
 // e3x_api.hip: set the message e3x_last_error() returns on this thread; returns code
 int report_error(int code, const char *msg);
 

@@ -43,6 +43,7 @@ struct PcgArgs {
     const T *minv;
     const void *hphase;      // homogeneous phase of each node, all ones for the others
     int wide;                // hphase holds uint32 ids (else uint16)
+    int64_t mstride;         // batch: nodes between the members' M^-1 and hphase arrays (0: shared)
     const T *minv_hom;       // [nphase][6]: M^-1 of a homogeneous node of each phase
     double *partials;
     PcgState *st;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs<T> A)
             T *g = A.g + o, *z = A.z + o, *h = A.h + o;
             const double g0 = g[0], g1 = g[1], g2 = g[2];
             double z0, z1, z2;
-            precond_apply(A.minv, ns, m, g0, g1, g2, z0, z1, z2);
+            precond_apply(A.minv + 6 * r * A.mstride, ns, m, g0, g1, g2, z0, z1, z2);
             const T t0 = (T)z0, t1 = (T)z1, t2 = (T)z2;
             z[0] = t0; z[1] = t1; z[2] = t2;
             if (A.kkk == 1) { h[0] = t0; h[1] = t1; h[2] = t2; }
@@ -154,8 +155,9 @@ __global__ __launch_bounds__(kBlock) void k_hah_stage(const double *partials, in
 // g -= lambda Ah (unless the true gradient replaces g this iteration); z = M^-1 g;
 // partials of z.g (f90:1919-1946).  DO_U: u -= lambda h here (iterations whose true
 // gradient needs the current u); otherwise the next iteration's apply moves u.  GSQ:
-// also the partials of g.g, in slots [NR + r] (E3X_STOP_NIST_ABS).
-template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z, bool GSQ = false>
+// also the partials of g.g, in slots [NR + r] (E3X_STOP_NIST_ABS).  MEM: the right-hand
+// sides are the members of a batch, each with its own M^-1 (read per right-hand side).
+template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z, bool GSQ = false, bool MEM = false>
 __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__restrict__ u, T *__restrict__ gv,
                                                 const T *__restrict__ hv, T *__restrict__ zv,
                                                 const T *__restrict__ ah, const T *__restrict__ minv)
@@ -176,23 +178,26 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
 #pragma unroll
     for (int q = 0; q < (GSQ ? 2 * NR : NR); q++) part[q] = 0.0;
     for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < ns; m += (int64_t)gridDim.x * kBlock) {
-        // the node's M^-1 block, once for all right-hand sides: from the per-phase table
-        // for a homogeneous node (the same bits as its entry in minv), else from minv
+        // the node's M^-1 block: from the per-phase table for a homogeneous node (the same
+        // bits as its entry in minv), else from minv; once for all right-hand sides, or per
+        // member of a batch
         double mi[6];
-        if constexpr (DO_Z) {
-            const int hp = hom_phase_at(A.hphase, A.wide, m);
+        auto load_mi = [&](int64_t n) {
+            const int hp = hom_phase_at(A.hphase, A.wide, n);
             if (hp >= 0) {
 #pragma unroll
                 for (int q = 0; q < 6; q++) mi[q] = (double)A.minv_hom[6 * hp + q];
             } else {
 #pragma unroll
-                for (int q = 0; q < 6; q++) mi[q] = (double)load_once(&minv[6 * m + q]);
+                for (int q = 0; q < 6; q++) mi[q] = (double)load_once(&minv[6 * n + q]);
             }
-        }
+        };
+        if constexpr (DO_Z && !MEM) load_mi(m);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             if (!act[r]) continue;
             const int64_t o = ((int64_t)r * ns + m) * 3;
+            if constexpr (DO_Z && MEM) load_mi(r * A.mstride + m);
             if constexpr (DO_U) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
@@ -228,10 +233,10 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
     if constexpr (DO_Z) write_partials<GSQ ? 2 * NR : NR>(part, A.partials);
 }
 
-template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z, bool GSQ = false>
+template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z, bool GSQ = false, bool MEM = false>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(PcgArgs<T> A)
 {
-    pcg_update_body<T, NR, UPDATE_G, DO_U, DO_Z, GSQ>(A, A.u, A.g, A.h, A.z, A.ah, A.minv);
+    pcg_update_body<T, NR, UPDATE_G, DO_U, DO_Z, GSQ, MEM>(A, A.u, A.g, A.h, A.z, A.ah, A.minv);
 }
 
 // gg, rel_res, history, divergence reset or gamma, stop test (f90:1938-2007); block r
@@ -344,6 +349,7 @@ PcgArgs<T> pcg_args(const Solver &s, int it)
     A.partials = s.partials; A.st = s.st; A.hist = s.hist;
     A.hphase = s.hphase;
     A.wide = s.wide ? 1 : 0;
+    A.mstride = s.mstride();
     if constexpr (sizeof(T) == 4) {
         A.g = s.g32; A.h = s.h32; A.z = s.z32; A.ah = s.ah32; A.minv = s.minv32; A.minv_hom = s.minv_hom32;
     } else {
@@ -389,20 +395,36 @@ int update_grid(const Solver &s)
     return (int)(g < cap ? g : cap);
 }
 
+// One launch of the streaming update in the handle's form: with the g.g partials (the NIST
+// stop rule) or not, and the batch form (per-member M^-1) or not.
+template <typename T, int NR, bool UG, bool DU, bool DZ>
+void launch_update(const Solver &s, dim3 vg, const PcgArgs<T> &A)
+{
+    const dim3 blk(kBlock);
+    const bool gsq = DZ && s.stop_rule == 1;
+    if (s.members) {
+        if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, UG, DU, DZ, true, true>), vg, blk, 0, s.stream, A);
+        else hipLaunchKernelGGL((k_pcg_update<T, NR, UG, DU, DZ, false, true>), vg, blk, 0, s.stream, A);
+    } else {
+        if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, UG, DU, DZ, true>), vg, blk, 0, s.stream, A);
+        else hipLaunchKernelGGL((k_pcg_update<T, NR, UG, DU, DZ>), vg, blk, 0, s.stream, A);
+    }
+}
+
 template <typename T, int NR>
 void iter_nr(Solver &s, int it, int replace_every)
 {
-    const int g = update_grid<T, NR>(s);
-    const dim3 vg(g), blk(kBlock);
+    // a batch keeps the streaming grid of a single solve (the one-rhs grid is never capped),
+    // so that every member's sums run over the same blocks as alone: the same bits
+    const int g = s.members ? (int)vector_grid(s) : update_grid<T, NR>(s);
+    const dim3 vg(g);
     const bool replace = replace_every > 0 && it % replace_every == 0;
     // 1. direction, the pending u steps, Ah = A h, the curvature (f90:1578-1907)
     (void)launch_apply_hah(s, sizeof(T) == 4, it, it >= 2 ? u_steps_of_apply(it, replace_every) : 0);
     PcgArgs<T> A = pcg_args<T>(s, it);
     // 2. updates, z and z.g (f90:1914-2007)
-    const bool gsq = s.stop_rule == 1;
     if (!replace) {
-        if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true, true>), vg, blk, 0, s.stream, A);
-        else hipLaunchKernelGGL((k_pcg_update<T, NR, true, false, true>), vg, blk, 0, s.stream, A);
+        launch_update<T, NR, true, false, true>(s, vg, A);
     } else {
         // reliable update: u first (the pending step of iteration it-1, if any, then this
         // iteration's), then the true gradient g = K u + b in FP64, then z
@@ -414,10 +436,9 @@ void iter_nr(Solver &s, int it, int replace_every)
             A.hm1 = (it & 1) ? s.h_alt : s.h;
         }
         A.usteps = it - u_done_after_apply(it, replace_every);
-        hipLaunchKernelGGL((k_pcg_update<T, NR, false, true, false>), vg, blk, 0, s.stream, A);
+        launch_update<T, NR, false, true, false>(s, vg, A);
         (void)launch_resid(s);
-        if (gsq) hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true, true>), vg, blk, 0, s.stream, A);
-        else hipLaunchKernelGGL((k_pcg_update<T, NR, false, false, true>), vg, blk, 0, s.stream, A);
+        launch_update<T, NR, false, false, true>(s, vg, A);
     }
     hipLaunchKernelGGL(k_update_finish, dim3(NR), dim3(kFinBlock), 0, s.stream, (const double *)s.partials, g, s.st,
                        s.hist, it, NR);

@@ -17,6 +17,7 @@ struct PostArgs {
     int64_t ns;
     const void *pix;         // 0-based phase ids (a handle's copy), uint16 or (wide) uint32
     int wide;
+    int64_t mstride;         // batch: nodes between the members' phase maps (0: one shared map)
     const int32_t *pix32;    // DEV form: the caller's labels 1..nphase
     int nphase;              // DEV form: labels outside [1, nphase] give NaN
     const double *cmod, *u;  // u: [nrhs][ns][3], DEV form [nrhs][3][ns] (u(ns,3) per rhs)
@@ -61,7 +62,7 @@ __device__ void voxel_strain_stress(const PostArgs &A, int r, int64_t m, double 
             return;
         }
     } else {
-        p = phase_at(A.pix, A.wide, m);
+        p = phase_at(A.pix, A.wide, r * A.mstride + m);
     }
     const double *c = A.cmod + (int64_t)p * 36;
     for (int q = 0; q < 6; q++) {
@@ -152,7 +153,7 @@ PostArgs post_args(const Solver &s)
 {
     PostArgs A{};
     A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.nrhs = s.nrhs; A.ns = s.ns;
-    A.pix = s.pix; A.wide = s.wide; A.cmod = s.cmod; A.u = s.u;
+    A.pix = s.pix; A.wide = s.wide; A.mstride = s.mstride(); A.cmod = s.cmod; A.u = s.u;
     for (int q = 0; q < 6 * kMaxRhs; q++) A.e[q] = s.strain[q];
     return A;
 }

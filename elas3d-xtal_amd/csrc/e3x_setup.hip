@@ -201,6 +201,7 @@ struct JumpArgs {
     int64_t ns;
     const void *pix;  // phase ids, uint16 or (wide) uint32
     int wide;
+    int64_t mstride;  // batch: nodes between the members' phase maps (0: one shared map)
     const double *dk;
     double e[6 * kMaxRhs];
 };
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *p
                 const int fx = vi == A.nx - 1, fy = vj == A.ny - 1, fz = vk == A.nz - 1;
                 if (!(fx | fy | fz)) continue;
                 const int64_t v = vi + (int64_t)A.nx * (vj + (int64_t)A.ny * vk);
-                const double *D = A.dk + (int64_t)phase_at(A.pix, A.wide, v) * 576;
+                const double *D = A.dk + (int64_t)phase_at(A.pix, A.wide, r * A.mstride + v) * 576;
                 for (int nn = 0; nn < 3; nn++) {
                     double sv = 0.0;
                     for (int m3 = 0; m3 < 3; m3++)
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(JumpArgs A, double *b, double *p
             // constant of the voxel whose lower corner is m
             const int fx = i == A.nx - 1, fy = j == A.ny - 1, fz = k == A.nz - 1;
             if (fx | fy | fz) {
-                const double *D = A.dk + (int64_t)phase_at(A.pix, A.wide, m) * 576;
+                const double *D = A.dk + (int64_t)phase_at(A.pix, A.wide, r * A.mstride + m) * 576;
                 double c = 0.0;
                 for (int nn = 0; nn < 3; nn++)
                     for (int mm = 0; mm < 8; mm++) {
@@ -413,7 +414,7 @@ JumpArgs jump_args(const Solver &s)
 {
     JumpArgs A{};
     A.nx = s.nx; A.ny = s.ny; A.nz = s.nz; A.nrhs = s.nrhs; A.ns = s.ns;
-    A.pix = s.pix; A.wide = s.wide; A.dk = s.dk;
+    A.pix = s.pix; A.wide = s.wide; A.mstride = s.mstride(); A.dk = s.dk;
     for (int q = 0; q < 6 * kMaxRhs; q++) A.e[q] = s.strain[q];
     return A;
 }
@@ -422,9 +423,16 @@ JumpArgs jump_args(const Solver &s)
 
 hipError_t launch_femat(Solver &s, double c11, double c12, double c44, const double *pm)
 {
-    hipLaunchKernelGGL(k_femat, dim3(s.nphase), dim3(kBlock), 0, s.stream, s.nphase, s.flag_m,
-                       s.orient, c11, c12, c44, pm[0], pm[1], pm[2], pm[3], s.cmod, s.dk, s.sten,
-                       s.dk32, s.sten32);
+    return launch_femat_range(s, 0, s.nphase, s.orient, c11, c12, c44, pm);
+}
+
+hipError_t launch_femat_range(Solver &s, int base, int nphase, const double *orient, double c11, double c12,
+                              double c44, const double *pm)
+{
+    const int64_t b = base;
+    hipLaunchKernelGGL(k_femat, dim3(nphase), dim3(kBlock), 0, s.stream, nphase, s.flag_m, orient, c11, c12, c44,
+                       pm[0], pm[1], pm[2], pm[3], s.cmod + 36 * b, s.dk + 576 * b, s.sten + kStenHalf * b,
+                       s.dk32 + 576 * b, s.sten32 + kStenHalf * b);
     return hipGetLastError();
 }
 
@@ -450,9 +458,14 @@ hipError_t launch_affine(Solver &s)
 hipError_t launch_precond(Solver &s, bool want32)
 {
     const int identity = s.pcg_variant == 2 ? 1 : 0;
-    hipLaunchKernelGGL(k_precond, dim3((unsigned)vector_blocks(s)), dim3(kBlock), 0, s.stream, s.nx, s.ny,
-                       s.nz, s.ns, (const void *)s.pix, s.wide ? 1 : 0, (const double *)s.dk, s.minv,
-                       want32 ? s.minv32 : nullptr, s.hphase, identity);
+    const int64_t idb = s.wide ? 4 : 2;  // phase-id bytes
+    for (int64_t mb = 0; mb < s.member_count(); mb++) {  // each member's microstructure
+        const int64_t n0 = mb * s.ns;
+        hipLaunchKernelGGL(k_precond, dim3((unsigned)vector_blocks(s)), dim3(kBlock), 0, s.stream, s.nx, s.ny,
+                           s.nz, s.ns, (const void *)((const char *)s.pix + idb * n0), s.wide ? 1 : 0,
+                           (const double *)s.dk, s.minv + 6 * n0, want32 ? s.minv32 + 6 * n0 : nullptr,
+                           (void *)((char *)s.hphase + idb * n0), identity);
+    }
     hipLaunchKernelGGL(k_precond_table, dim3((unsigned)((s.nphase + 63) / 64)), dim3(64), 0, s.stream, s.nphase,
                        (const double *)s.dk, s.minv_hom, want32 ? s.minv_hom32 : nullptr, identity);
     s.precond_identity = s.pcg_variant == 2;

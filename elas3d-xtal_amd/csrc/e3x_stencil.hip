@@ -54,6 +54,7 @@ struct StencilArgs {
     int64_t ns;
     const void *pix;       // phase ids: uint16, or uint32 on a wide handle (the kernel's PhT)
     const PermT *perm;     // lane order per tile (Solver::perm)
+    int64_t mstride;       // batch: nodes between the members' phase maps (0: one shared map)
     const T *dk, *sten;
     const T *x;            // [nrhs][ns][3]
     T *y;                  // [nrhs][ns][3] (kApply, kEnergy, kPlain)
@@ -153,6 +154,8 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     const int64_t ns = A.ns, o = (int64_t)(3 * r) * ns;
+    // a batch's member r has its own phase map and lane order
+    const int64_t mbase = (int64_t)r * A.mstride, ptile = A.mstride != 0 ? (int64_t)r * ntile * kSB : 0;
     const int tid = threadIdx.x;
     if constexpr (HAS_ST) {
         // a launch the host expects to find its right-hand side stopped checks before it loads
@@ -186,11 +189,11 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
     for (int k = 0; k < kVoxPer; k++) {
         const int q = min(tid + k * kSB, VN - 1);
         const HPos p = {q % VX, (q / VX) % VY, q / (VX * VY)};
-        pvl[k] = (int)pix[gpos(p)];
+        pvl[k] = (int)pix[mbase + gpos(p)];
     }
     // this lane's node: t in the tile's wave-cube numbering (homogeneous nodes by phase,
     // then mixed ones by their phases; k_tile_order)
-    const int t = A.perm[(int64_t)bid * kSB + tid];
+    const int t = A.perm[ptile + (int64_t)bid * kSB + tid];
     // halo: the tile's own node tid and shell positions tid + k kSB (a thread without
     // one loads its own node again and does not store it)
     const HPos pin = inner_pos(tid);
@@ -593,6 +596,7 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
     A.d_nty = fast_div((unsigned)A.nty);
     A.wrap1 = s.nx >= 8 && s.ny >= 8 && s.nz >= 8;
     A.ns = s.ns; A.pix = s.pix; A.perm = s.perm; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
+    A.mstride = s.mstride();
     A.partials = s.partials; A.st = s.st;
     for (int r = 0; r < kMaxRhs; r++) A.rmap[r] = r;
     return A;
@@ -710,16 +714,18 @@ hipError_t launch_apply_only(Solver &s, bool mixed, int usteps)
     return hipGetLastError();
 }
 
-hipError_t launch_tile_order(Solver &s)
+hipError_t launch_tile_order(Solver &s, int member)
 {
     const dim3 g((unsigned)stencil_blocks(s)), b(kSB);
     const int ntx = (s.nx + TX - 1) / TX, nty = (s.ny + TY - 1) / TY;
+    const int64_t mb = (int64_t)member * s.mstride();                    // member's phase map
+    PermT *perm = s.perm + (s.members ? (int64_t)member * stencil_blocks(s) * kSB : 0);
     if (s.wide)
         hipLaunchKernelGGL(k_tile_order<uint32_t>, g, b, 0, s.stream, s.nx, s.ny, s.nz, ntx, nty,
-                           (const uint32_t *)s.pix, s.perm);
+                           (const uint32_t *)s.pix + mb, perm);
     else
         hipLaunchKernelGGL(k_tile_order<uint16_t>, g, b, 0, s.stream, s.nx, s.ny, s.nz, ntx, nty,
-                           (const uint16_t *)s.pix, s.perm);
+                           (const uint16_t *)s.pix + mb, perm);
     return hipGetLastError();
 }
 

@@ -5,7 +5,7 @@ this package only binds it.
 """
 from . import _lib
 from .solver import (AML_DEFAULT, LDEMB_DEFAULT, TOL_DEFAULT, Elas3D, SolveResult, effective_stiffness, solve,
-                     unit_strain)
+                     solve_batch, unit_strain)
 
-__all__ = ["Elas3D", "SolveResult", "solve", "effective_stiffness", "unit_strain", "TOL_DEFAULT", "AML_DEFAULT",
+__all__ = ["Elas3D", "SolveResult", "solve", "solve_batch", "effective_stiffness", "unit_strain", "TOL_DEFAULT", "AML_DEFAULT",
            "LDEMB_DEFAULT", "_lib"]

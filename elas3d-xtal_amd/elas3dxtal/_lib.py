@@ -39,6 +39,9 @@ SIGNATURES = {
     "e3x_device_count": (C.c_int, [_ip]),
     "e3x_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "e3x_destroy": (C.c_int, [_vp]),
+    "e3x_create_batch": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int, C.POINTER(_vp)]),
+    "e3x_set_member": (C.c_int, [_vp, C.c_int, _i32p, _dp, C.c_double, C.c_double, C.c_double, _dp, C.c_int,
+                                 _dp]),
     "e3x_memory_bytes": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "e3x_set_microstructure": (C.c_int, [_vp, _i32p, _dp]),
     "e3x_assig": (C.c_int, [_vp, _dp]),

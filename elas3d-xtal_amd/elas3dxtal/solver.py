@@ -28,17 +28,28 @@ def unit_strain(k, magnitude=AML_DEFAULT):
 
 
 class Elas3D:
-    """One microstructure on one GPU; nrhs load cases solved together."""
+    """One microstructure on one GPU; nrhs load cases solved together.  With
+    batch_nphase (a list of 2..6 phase counts) the handle is instead a batch of that many
+    separate problems on one grid (e3x_create_batch): right-hand side r is member r, set
+    by set_member."""
 
-    def __init__(self, shape, nphase, nrhs=1, device=0):
+    def __init__(self, shape, nphase=None, nrhs=1, device=0, batch_nphase=None):
         lib = L.load()
         self.shape = tuple(int(v) for v in shape)
         self.nx, self.ny, self.nz = self.shape
         self.ns = self.nx * self.ny * self.nz
-        self.nphase = int(nphase)
-        self.nrhs = int(nrhs)
         h = C.c_void_p()
-        L.check(lib.e3x_create(self.nx, self.ny, self.nz, self.nphase, self.nrhs, device, C.byref(h)))
+        if batch_nphase is not None:
+            bn = np.ascontiguousarray(batch_nphase, dtype=np.int32)
+            self.nphase = int(bn.sum())
+            self.nrhs = int(bn.size)
+            self.batch_nphase = bn.copy()
+            L.check(lib.e3x_create_batch(self.nx, self.ny, self.nz, self.nrhs, L.iptr(bn), device, C.byref(h)))
+        else:
+            self.nphase = int(nphase)
+            self.nrhs = int(nrhs)
+            self.batch_nphase = None
+            L.check(lib.e3x_create(self.nx, self.ny, self.nz, self.nphase, self.nrhs, device, C.byref(h)))
         self._h = h
         self._lib = lib
         self.strain6 = np.zeros((self.nrhs, 6))
@@ -85,6 +96,21 @@ class Elas3D:
                                                f"3*(nphase-1) = {3 * (self.nphase - 1)}")
         L.check(self._lib.e3x_set_microstructure(self._h, L.i32ptr(pix), L.dptr(ori)))
         self._pix_max = int(pix.max())
+
+    def set_member(self, member, pix, orientation, c11, c12, c44, strain6, phasemod=(0.0, 0.0), flag_m=1):
+        """Member `member` of a batch: microstructure, materials and load (e3x_set_member)."""
+        pix = np.ascontiguousarray(pix, dtype=np.int32).ravel()
+        if pix.size != self.ns:
+            raise L.E3xError(L.E3X_EINVAL, f"pix size {pix.size} does not match ns {self.ns}")
+        ori = None if orientation is None else np.ascontiguousarray(orientation, dtype=np.float64).ravel()
+        pm = np.zeros(4)
+        pmv = np.asarray(phasemod, float).ravel()
+        pm[:pmv.size] = pmv
+        e = np.ascontiguousarray(np.asarray(strain6, float).ravel()[:6])
+        L.check(self._lib.e3x_set_member(self._h, member, L.i32ptr(pix), L.dptr(ori), c11, c12, c44, L.dptr(pm),
+                                         flag_m, L.dptr(e)))
+        self.strain6[member] = e
+        self._pix_max = max(self._pix_max, int(pix.max()))
 
     def assig(self):
         prob = np.empty(self.nphase)
@@ -282,6 +308,35 @@ def solve(pix, orientation, shape, nphase, c11, c12, c44, strain6, phasemod=(0.0
         res = SolveResult(utot0=ut0, gg0=gg0, utot=ut1, gg=d["gg"], rel_res=np.sqrt(d["gg"] / gg0),
                           lstep=d["lstep"], exit=d["exit"], reset_count=d["reset_count"], C=cval,
                           stress6=st, strain6=sn, history=d["history"], prob=prob,
+                          timing=dict(precond_ms=d["precond_ms"], pcg_ms=d["pcg_ms"]))
+    except Exception:
+        s.close()
+        raise
+    if keep:
+        return res, s
+    s.close()
+    return res
+
+
+def solve_batch(shape, members, tol=TOL_DEFAULT, ldemb=LDEMB_DEFAULT, precision=L.PREC_FP64, replace_every=50,
+                device=0, keep=False):
+    """Several problems on one grid solved as one batch (e3x_create_batch): the main
+    program's sequence (f90:326-540) for every member at once.  members: list of dicts
+    with pix, orientation, nphase, c11, c12, c44, strain6 and optionally phasemod, flag_m.
+    Returns SolveResult with one row per member (and the live Elas3D if keep=True)."""
+    s = Elas3D(shape, batch_nphase=[int(m["nphase"]) for m in members], device=device)
+    try:
+        for r, m in enumerate(members):
+            s.set_member(r, m["pix"], m.get("orientation"), m["c11"], m["c12"], m["c44"], m["strain6"],
+                         phasemod=m.get("phasemod", (0.0, 0.0)), flag_m=m.get("flag_m", 1))
+        s.affine_guess()
+        ut0, gg0 = s.energy()
+        d = s.dembx(gg0, tol=tol, ldemb=ldemb, kkk=1, precision=precision, replace_every=replace_every)
+        ut1, _ = s.energy()
+        st, sn = s.stress()
+        res = SolveResult(utot0=ut0, gg0=gg0, utot=ut1, gg=d["gg"], rel_res=np.sqrt(d["gg"] / gg0),
+                          lstep=d["lstep"], exit=d["exit"], reset_count=d["reset_count"], C=np.zeros(s.nrhs),
+                          stress6=st, strain6=sn, history=d["history"],
                           timing=dict(precond_ms=d["precond_ms"], pcg_ms=d["pcg_ms"]))
     except Exception:
         s.close()

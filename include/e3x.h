@@ -108,6 +108,32 @@ int e3x_device_count(int *count);
  */
 int e3x_create(int nx, int ny, int nz, int nphase, int nrhs, int device, e3x_solver **out);
 
+/*
+ * A batch of `members` (2..6) separate problems on one nx*ny*nz grid, solved together.  No
+ * counterpart in the reference, which solves one problem per run: it is how the serving
+ * loop co-batches small grids.  Right-hand side r of the handle is member r, with its own
+ * microstructure (nphase[r] phases), materials and load, set by e3x_set_member; the
+ * members share every launch (the stencil grid has a row per member, the streaming
+ * kernels take them in turn), so a few small grids fill the GPU.  Once every member is
+ * set, e3x_affine_guess, e3x_energy, e3x_dembx(_ex) / _begin / _advance / _end,
+ * e3x_stress, e3x_stress_fullfield, e3x_hotspots, e3x_get_field and e3x_set_field take
+ * the members as they take right-hand sides, and each member's results are bit for bit
+ * those of its problem solved alone (the same kernels, grids and order of every sum).
+ * e3x_set_microstructure, e3x_assig, e3x_femat and e3x_effective_stiffness are not for
+ * batch handles (E3X_ESTATE).
+ */
+int e3x_create_batch(int nx, int ny, int nz, int members, const int *nphase, int device, e3x_solver **out);
+
+/*
+ * Member r of a batch: its microstructure (pix labels in [1, nphase[r]] and Rodrigues
+ * orientation[3*(nphase[r]-1)], as e3x_set_microstructure), materials and load (as
+ * e3x_femat with one strain6).  Builds the member's tables and lane order; the call that
+ * sets the last member also builds every member's jump right-hand side and constant
+ * (f90:815-1100).  Setting a member again replaces it (the others are kept).
+ */
+int e3x_set_member(e3x_solver *s, int member, const int32_t *pix, const double *orientation, double c11,
+                   double c12, double c44, const double *phasemod, int flag_m, const double *strain6);
+
 /* Free all device memory of the handle (f90:543-549). */
 int e3x_destroy(e3x_solver *s);
 
