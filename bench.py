@@ -180,7 +180,7 @@ def serving_requests(n_requests, seed=5000):
     return reqs
 
 
-def run_serving(reqs, interarrival_ms, max_active, seed=5001):
+def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
     """Serving stream over `reqs` with Poisson arrivals (mean gap interarrival_ms; 0: all
     at once).  One host thread submits at the arrival times and drives the server between
     them; latency is arrival to result."""
@@ -190,7 +190,7 @@ def run_serving(reqs, interarrival_ms, max_active, seed=5001):
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(interarrival_ms * 1e-3, n_requests) if interarrival_ms > 0 else np.zeros(n_requests)
     arrivals = np.cumsum(gaps) - gaps[0]
-    with Server(max_active=max_active) as sv:
+    with Server(max_active=max_active, batch_members=batch_members) as sv:
         t0 = time.perf_counter()
         done = []
         k = 0
@@ -209,8 +209,15 @@ def run_serving(reqs, interarrival_ms, max_active, seed=5001):
     lat = np.array([r["t_done"] - r["t_submit"] for r in done])
     ok = sum(r["status"] == 0 and r["exit"] == L.PCG_CONVERGED for r in done)
     dofups = sum(3.0 * np.prod(r["shape"]) * r["lstep"] for r in done)
+    by_edge = {}
+    for e in sorted({r["shape"][0] for r in done}):
+        le = np.array([r["t_done"] - r["t_submit"] for r in done if r["shape"][0] == e])
+        by_edge[str(e)] = dict(n=int(le.size), p50_ms=round(float(np.percentile(le, 50) * 1e3), 1),
+                               p99_ms=round(float(np.percentile(le, 99) * 1e3), 1))
     return {
         "requests": n_requests, "converged": int(ok), "max_active": max_active,
+        "co_batching_members": batch_members,
+        "latency_by_edge": by_edge,
         "mean_interarrival_ms": interarrival_ms,
         "solves_per_s": n_requests / wall,
         "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
@@ -288,7 +295,8 @@ def main():
     ap.add_argument("--six-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=256, help="serving stream length (0: skip)")
     ap.add_argument("--serve-interarrival-ms", type=float, default=70.0)
-    ap.add_argument("--serve-max-active", type=int, default=8)
+    ap.add_argument("--serve-max-active", type=int, default=3)
+    ap.add_argument("--serve-batch-members", type=int, default=6, help="co-batching of small grids (1: off)")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run of the apply kernel and a calibration copy, for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -388,7 +396,8 @@ def main():
         out["six_rhs"] = six
     if args.serve_requests > 0:
         reqs = serving_requests(args.serve_requests)
-        out["serving"] = run_serving(reqs, args.serve_interarrival_ms, args.serve_max_active)
+        out["serving"] = run_serving(reqs, args.serve_interarrival_ms, args.serve_max_active,
+                                     batch_members=args.serve_batch_members)
         del reqs
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, c11, c12, c44, e6, args.cpu_iters)

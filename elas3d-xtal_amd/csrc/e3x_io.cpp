@@ -14,6 +14,10 @@
 #include <vector>
 
 #include "e3x.h"
+#include "e3x_serve.h"
+
+#include <cstdlib>
+#include <map>
 
 namespace {
 
@@ -257,6 +261,125 @@ void matrix_to_bunge(const double g[3][3], double e[3])
 }
 
 }  // namespace
+
+// ------------------------------------------------------------------ JSON controls
+// The controls of a serving request: one flat JSON object whose values are numbers,
+// strings, booleans, null or arrays of numbers.
+struct JVal {
+    enum Kind { kNum, kStr, kBool, kNull, kArr } kind = kNull;
+    double num = 0;
+    std::string str;
+    std::vector<double> arr;
+};
+
+struct JParser {
+    const char *p;
+    std::string err;
+    void ws()
+    {
+        while (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r') p++;
+    }
+    bool str(std::string &out)
+    {
+        if (*p != '"') return err = "expected a string", false;
+        p++;
+        out.clear();
+        while (*p && *p != '"') {
+            if (*p == '\\') {
+                p++;
+                if (!*p) break;
+                const char c = *p;
+                out += c == 'n' ? '\n' : c == 't' ? '\t' : c;  // \" \\ \/ and the rest as written
+                p++;
+                continue;
+            }
+            out += *p++;
+        }
+        if (*p != '"') return err = "unterminated string", false;
+        p++;
+        return true;
+    }
+    bool num(double &out)
+    {
+        char *end = nullptr;
+        out = std::strtod(p, &end);
+        if (end == p) return err = "expected a number", false;
+        p = end;
+        return true;
+    }
+    bool value(JVal &v)
+    {
+        ws();
+        if (*p == '"') {
+            v.kind = JVal::kStr;
+            return str(v.str);
+        }
+        if (*p == '[') {
+            p++;
+            v.kind = JVal::kArr;
+            ws();
+            if (*p == ']') {
+                p++;
+                return true;
+            }
+            while (true) {
+                ws();
+                double d;
+                if (!num(d)) return false;
+                v.arr.push_back(d);
+                ws();
+                if (*p == ',') {
+                    p++;
+                    continue;
+                }
+                if (*p == ']') {
+                    p++;
+                    return true;
+                }
+                return err = "expected , or ] in an array", false;
+            }
+        }
+        if (!std::strncmp(p, "true", 4)) { p += 4; v.kind = JVal::kBool; v.num = 1; return true; }
+        if (!std::strncmp(p, "false", 5)) { p += 5; v.kind = JVal::kBool; v.num = 0; return true; }
+        if (!std::strncmp(p, "null", 4)) { p += 4; v.kind = JVal::kNull; return true; }
+        v.kind = JVal::kNum;
+        return num(v.num);
+    }
+    bool object(std::map<std::string, JVal> &out)
+    {
+        ws();
+        if (*p != '{') return err = "expected a JSON object", false;
+        p++;
+        ws();
+        if (*p == '}') {
+            p++;
+            return true;
+        }
+        while (true) {
+            ws();
+            std::string key;
+            if (!str(key)) return false;
+            ws();
+            if (*p != ':') return err = "expected : after a key", false;
+            p++;
+            JVal v;
+            if (!value(v)) return false;
+            out[key] = v;
+            ws();
+            if (*p == ',') {
+                p++;
+                continue;
+            }
+            if (*p == '}') {
+                p++;
+                ws();
+                if (*p) return err = "text after the object", false;
+                return true;
+            }
+            return err = "expected , or } in the object", false;
+        }
+    }
+};
 
 extern "C" {
 
@@ -537,6 +660,127 @@ int e3x_generate(const e3x_gen_params *p, int32_t *pix, double *rod, double *eul
     }
     if (pore_out) *pore_out = (double)npore / (double)ns;
     return E3X_OK;
+}
+
+int e3x_request_from_files(const char *h5_path, const char *json, e3x_request *rq, int32_t **pix,
+                           double **orientation, char *fullfield_path, int path_cap)
+{
+    if (!h5_path || !rq || !pix || !orientation) return fail(E3X_EINVAL, "NULL argument");
+    *pix = nullptr;
+    *orientation = nullptr;
+    if (fullfield_path && path_cap > 0) fullfield_path[0] = 0;
+    std::map<std::string, JVal> c;
+    if (json && *json) {
+        JParser jp{json, {}};
+        if (!jp.object(c)) return fail(E3X_EINVAL, "JSON controls: " + jp.err);
+    }
+    static const char *known[] = {"grid", "c11", "c12", "c44", "zener", "strain6", "phasemod", "flag_m",
+                                  "precision", "replace_every", "tol", "ldemb", "fullfield"};
+    for (const auto &kv : c) {
+        bool ok = false;
+        for (const char *k : known) ok |= kv.first == k;
+        if (!ok) return fail(E3X_EINVAL, "JSON controls: unknown key \"" + kv.first + "\"");
+    }
+    auto num = [&](const char *k, double dflt, double *out) -> int {
+        auto it = c.find(k);
+        if (it == c.end() || it->second.kind == JVal::kNull) { *out = dflt; return E3X_OK; }
+        if (it->second.kind != JVal::kNum && it->second.kind != JVal::kBool)
+            return fail(E3X_EINVAL, std::string("JSON controls: ") + k + " must be a number");
+        *out = it->second.num;
+        return E3X_OK;
+    };
+    auto arr = [&](const char *k, size_t n, std::vector<double> *out) -> int {
+        auto it = c.find(k);
+        if (it == c.end()) return E3X_OK;
+        if (it->second.kind != JVal::kArr || it->second.arr.size() != n)
+            return fail(E3X_EINVAL, std::string("JSON controls: ") + k + " must be an array of " + std::to_string(n) +
+                                        " numbers");
+        *out = it->second.arr;
+        return E3X_OK;
+    };
+    // the grid: "grid", else the /pix dimensions attribute the MATLAB generator writes
+    int64_t npix = 0, norient = 0;
+    int32_t dims[3] = {0, 0, 0};
+    if (int rc = e3x_h5_probe_input(h5_path, &npix, &norient, dims)) return rc;
+    std::vector<double> grid;
+    if (int rc = arr("grid", 3, &grid)) return rc;
+    int nx = dims[0], ny = dims[1], nz = dims[2];
+    if (!grid.empty()) {
+        nx = (int)grid[0];
+        ny = (int)grid[1];
+        nz = (int)grid[2];
+    }
+    if (nx < 2 || ny < 2 || nz < 2)
+        return fail(E3X_EINVAL, "grid unknown: give \"grid\" (the file has no /pix dimensions attribute)");
+    const int64_t ns = (int64_t)nx * ny * nz;
+    if (norient % 3 != 0) return fail(E3X_EINVAL, "/orientation holds a number of values not divisible by 3");
+    const int nphase = (int)(norient / 3) + 1;  // n_grains + 1: the pore phase (f90:88)
+    // materials: Pa; "zener" A sets c11 = c12 + 2 c44 / A
+    double c11, c12, c44, zener;
+    if (int rc = num("c12", 147.3e9, &c12)) return rc;
+    if (int rc = num("c44", 124.7e9, &c44)) return rc;
+    if (int rc = num("zener", 0.0, &zener)) return rc;
+    if (int rc = num("c11", zener > 0 ? c12 + 2.0 * c44 / zener : 246.5e9, &c11)) return rc;
+    if (c.count("c11") && zener > 0) return fail(E3X_EINVAL, "JSON controls: give c11 or zener, not both");
+    std::vector<double> e6 = {(double)1.0e-3f, 0, 0, 0, 0, 0}, pm = {0, 0};  // aml = 1.0e-3 (f90:125)
+    if (int rc = arr("strain6", 6, &e6)) return rc;
+    if (c.count("phasemod")) {
+        auto it = c.find("phasemod");
+        if (it->second.kind != JVal::kArr || (it->second.arr.size() != 2 && it->second.arr.size() != 4))
+            return fail(E3X_EINVAL, "JSON controls: phasemod must hold 2 or 4 numbers");
+        pm = it->second.arr;
+    }
+    double flag_m, re, tol, ldemb;
+    if (int rc = num("flag_m", 1, &flag_m)) return rc;
+    if (int rc = num("replace_every", 50, &re)) return rc;
+    if (int rc = num("tol", (double)1.0e-8f, &tol)) return rc;  // tol = 1.0d-8 as real (f90:148)
+    if (int rc = num("ldemb", 10000, &ldemb)) return rc;        // f90:141
+    int precision = E3X_PREC_FP64;
+    if (auto it = c.find("precision"); it != c.end()) {
+        if (it->second.kind != JVal::kStr || (it->second.str != "fp64" && it->second.str != "mixed"))
+            return fail(E3X_EINVAL, "JSON controls: precision must be \"fp64\" or \"mixed\"");
+        precision = it->second.str == "mixed" ? E3X_PREC_MIXED : E3X_PREC_FP64;
+    }
+    if (auto it = c.find("fullfield"); it != c.end() && it->second.kind != JVal::kNull) {
+        if (it->second.kind != JVal::kStr) return fail(E3X_EINVAL, "JSON controls: fullfield must be a path");
+        if (!fullfield_path || (int)it->second.str.size() + 1 > path_cap)
+            return fail(E3X_EINVAL, "fullfield path does not fit the caller's buffer");
+        std::memcpy(fullfield_path, it->second.str.c_str(), it->second.str.size() + 1);
+    }
+    // the microstructure, with the reference's checks (f90:2879-2991)
+    int32_t *p = static_cast<int32_t *>(std::malloc(sizeof(int32_t) * (size_t)ns));
+    double *o = static_cast<double *>(std::malloc(sizeof(double) * (size_t)std::max<int64_t>(norient, 1)));
+    if (!p || !o) {
+        std::free(p);
+        std::free(o);
+        return fail(E3X_ENOMEM, "host allocation failed");
+    }
+    if (int rc = e3x_h5_read_input(h5_path, ns, nphase, p, o)) {
+        std::free(p);
+        std::free(o);
+        return rc;
+    }
+    std::memset(rq, 0, sizeof(*rq));
+    rq->nx = nx; rq->ny = ny; rq->nz = nz; rq->nphase = nphase;
+    rq->pix = p;
+    rq->orientation = nphase > 1 ? o : nullptr;
+    rq->c11 = c11; rq->c12 = c12; rq->c44 = c44;
+    for (size_t k = 0; k < pm.size(); k++) rq->phasemod[k] = pm[k];
+    rq->flag_m = (int)flag_m;
+    for (int k = 0; k < 6; k++) rq->strain6[k] = e6[(size_t)k];
+    rq->precision = precision;
+    rq->replace_every = precision == E3X_PREC_MIXED ? (int)re : 0;
+    rq->tol = (float)tol;
+    rq->ldemb = (int)ldemb;
+    *pix = p;
+    *orientation = o;
+    return E3X_OK;
+}
+
+void e3x_request_free(int32_t *pix, double *orientation)
+{
+    std::free(pix);
+    std::free(orientation);
 }
 
 }  // extern "C"

@@ -1,12 +1,15 @@
 // Request loop over the gfx950 solver (include/e3x_serve.h): continuous batching of
 // solves of different grid sizes on one GPU.
 //
-// Each active request owns a solver handle, hence a HIP stream; its PCG advances in
+// Each active solve owns a solver handle, hence a HIP stream; its PCG advances in
 // chunks (e3x_dembx_advance without waiting), so the kernels of all active solves are
-// in flight together and small grids fill the GPU side by side.  Admission is by the
-// device bytes a handle needs (first fit in arrival order under the budget and the slot
-// limit); a finished handle stays in a pool for a later request of the same grid size
-// (freeing device memory synchronizes the device, so it is done only to make room).
+// in flight together.  Queued requests of one small grid size (and the same precision
+// and PCG controls) are admitted together as one batch handle (e3x_create_batch): their
+// problems share every launch, which fills the GPU where one small grid alone leaves it
+// idle.  Admission is by the device bytes a handle needs (first fit in arrival order
+// under the budget and the slot limit); a finished handle stays in a pool for a later
+// request (or batch) of the same grid size (freeing device memory synchronizes the
+// device, so it is done only to make room).
 #include "e3x.h"
 #include "e3x_serve.h"
 #include "e3x_internal.hpp"
@@ -16,6 +19,7 @@
 #include <cstring>
 #include <deque>
 #include <list>
+#include <iterator>
 #include <new>
 #include <string>
 #include <thread>
@@ -40,13 +44,16 @@ struct Request {
 struct Handle {
     e3x_solver *h = nullptr;
     int nx = 0, ny = 0, nz = 0, nphase_cap = 0;
+    int members = 0;  // a batch handle's member count (0: a one-problem handle)
     int64_t bytes = 0;
 };
 
+// One solve on the GPU: a request on its own handle, or the members of a batch.
 struct Active {
-    Request req;
+    std::vector<Request> reqs;
     Handle hd;
-    double t_admit = 0, utot0 = 0;
+    double t_admit = 0;
+    double utot0[kMaxRhs] = {};
 };
 
 struct Server {
@@ -54,7 +61,9 @@ struct Server {
     int64_t budget = 0, used = 0;
     std::chrono::steady_clock::time_point t0;
     int64_t next_id = 0;
-    std::deque<Request> queue;
+    int batch_members = kMaxRhs;               // co-batching: at most this many per batch (1: off)
+    int64_t batch_max_nodes = 128 * 128 * 128; // ... of grids with at most this many nodes
+    std::list<Request> queue;
     std::vector<Active> active;
     std::list<Handle> pool;  // idle handles, least recently used first
     std::deque<e3x_response> done;
@@ -133,14 +142,20 @@ void make_room(Server &S, int64_t extra)
 // An idle handle of this grid size and enough phases, or one made room for and created.
 // Every path books the bytes the request will hold before it is admitted: a pooled
 // handle that lacks the mixed buffers a mixed request allocates is booked for them too.
-int acquire_handle(Server &S, const e3x_request &rq, int64_t need, Handle *out, bool *fits)
+int acquire_handle(Server &S, const e3x_request &rq, int members, int nphase_each, bool fields, int64_t need,
+                   Handle *out, bool *fits)
 {
     *fits = true;
     const bool mixed = rq.precision == E3X_PREC_MIXED;
+    const int cap = std::max(kPoolPhases, nphase_each) * std::max(members, 1);
     for (auto it = S.pool.begin(); it != S.pool.end(); ++it)
-        if (it->nx == rq.nx && it->ny == rq.ny && it->nz == rq.nz && it->nphase_cap >= rq.nphase) {
+        if (it->nx == rq.nx && it->ny == rq.ny && it->nz == rq.nz && it->members == members &&
+            it->nphase_cap >= cap) {
             Handle hd = *it;
-            const int64_t extra = (mixed && !has_mixed_buffers(hd.h)) ? mixed_bytes(rq.nx, rq.ny, rq.nz) : 0;
+            const int64_t ns = (int64_t)rq.nx * rq.ny * rq.nz;
+            const int64_t extra =
+                ((mixed && !has_mixed_buffers(hd.h)) ? mixed_bytes(rq.nx, rq.ny, rq.nz) * std::max(members, 1) : 0) +
+                ((fields && reinterpret_cast<Solver *>(hd.h)->post == nullptr) ? 7 * 8 * ns : 0);
             S.pool.erase(it);
             make_room(S, extra);
             if (S.used + extra > S.budget) {
@@ -161,8 +176,15 @@ int acquire_handle(Server &S, const e3x_request &rq, int64_t need, Handle *out, 
     }
     Handle hd;
     hd.nx = rq.nx; hd.ny = rq.ny; hd.nz = rq.nz;
-    hd.nphase_cap = std::max(kPoolPhases, rq.nphase);
-    if (int rc = e3x_create(rq.nx, rq.ny, rq.nz, hd.nphase_cap, 1, S.device, &hd.h)) return rc;
+    hd.nphase_cap = cap;
+    hd.members = members;
+    if (members == 0) {
+        if (int rc = e3x_create(rq.nx, rq.ny, rq.nz, hd.nphase_cap, 1, S.device, &hd.h)) return rc;
+    } else {
+        Solver *sv = nullptr;
+        if (int rc = solver_create(rq.nx, rq.ny, rq.nz, hd.nphase_cap, members, members, S.device, &sv)) return rc;
+        hd.h = reinterpret_cast<e3x_solver *>(sv);
+    }
     // booked at the admission estimate (which includes the mixed buffers and the history a
     // mixed request allocates later); retire_handle settles it to what the handle holds
     hd.bytes = std::max(handle_bytes(hd.h), need);
@@ -182,32 +204,93 @@ e3x_response make_response(const Request &r, int status)
     return o;
 }
 
-// The main program's sequence up to the PCG entry (f90:380-473) on the handle.
-int start_solve(Active &a)
+// Labels of the request inside [1, nphase] (f90:2986-2989), checked on the host before the
+// request joins a solve, so that one bad request does not fail a batch.
+bool labels_ok(const Request &r)
 {
-    const e3x_request &q = a.req.rq;
-    e3x_solver *h = a.hd.h;
-    if (int rc = solver_set_nphase(*reinterpret_cast<Solver *>(h), q.nphase)) return rc;
-    if (int rc = e3x_set_microstructure(h, a.req.pix.data(), a.req.ori.empty() ? nullptr : a.req.ori.data()))
-        return rc;
-    double cval = 0;
-    if (int rc = e3x_femat(h, q.c11, q.c12, q.c44, q.phasemod, q.flag_m, q.strain6, &cval)) return rc;
-    if (int rc = e3x_affine_guess(h)) return rc;
-    double gg0 = 0;
-    if (int rc = e3x_energy(h, &a.utot0, &gg0)) return rc;
-    return e3x_dembx_begin(h, q.precision, q.replace_every, &gg0, q.tol, q.ldemb, 1);
+    for (int32_t v : r.pix)
+        if (v < 1 || v > r.rq.nphase) return false;
+    return true;
 }
 
-// After the PCG (f90:479-519): energy and averages.
-int finish_solve(Active &a, e3x_response &o)
+// Requests that can share a batch: one grid, precision and set of PCG controls.
+bool same_batch(const e3x_request &a, const e3x_request &b)
+{
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.precision == b.precision &&
+           a.replace_every == b.replace_every && a.tol == b.tol && a.ldemb == b.ldemb && a.flag_m == b.flag_m;
+}
+
+// The main program's sequence up to the PCG entry (f90:380-473) on the handle: for one
+// request, or for every member of a batch.
+int start_solve(Active &a)
+{
+    const e3x_request &q = a.reqs[0].rq;
+    e3x_solver *h = a.hd.h;
+    Solver &sv = *reinterpret_cast<Solver *>(h);
+    const int n = (int)a.reqs.size();
+    double gg0[kMaxRhs] = {};
+    if (a.hd.members == 0) {
+        if (int rc = solver_set_nphase(sv, q.nphase)) return rc;
+        const Request &r = a.reqs[0];
+        if (int rc = e3x_set_microstructure(h, r.pix.data(), r.ori.empty() ? nullptr : r.ori.data())) return rc;
+        double cval = 0;
+        if (int rc = e3x_femat(h, q.c11, q.c12, q.c44, q.phasemod, q.flag_m, q.strain6, &cval)) return rc;
+    } else {
+        int np[kMaxRhs];
+        for (int k = 0; k < n; k++) np[k] = a.reqs[(size_t)k].rq.nphase;
+        if (int rc = solver_set_member_layout(sv, np)) return rc;
+        for (int k = 0; k < n; k++) {
+            const Request &r = a.reqs[(size_t)k];
+            const e3x_request &rk = r.rq;
+            if (int rc = e3x_set_member(h, k, r.pix.data(), r.ori.empty() ? nullptr : r.ori.data(), rk.c11, rk.c12,
+                                        rk.c44, rk.phasemod, rk.flag_m, rk.strain6))
+                return rc;
+        }
+    }
+    if (int rc = e3x_affine_guess(h)) return rc;
+    if (int rc = e3x_energy(h, a.utot0, gg0)) return rc;
+    return e3x_dembx_begin(h, q.precision, q.replace_every, gg0, q.tol, q.ldemb, 1);
+}
+
+// After the PCG (f90:479-519): energy and averages, one response per request.
+int finish_solve(Active &a, std::vector<e3x_response> &out)
 {
     e3x_solver *h = a.hd.h;
-    if (int rc = e3x_dembx_end(h, &o.lstep, &o.gg, nullptr, &o.exit_flag, &o.reset_count)) return rc;
-    double gg = 0;
-    if (int rc = e3x_energy(h, &o.utot, &gg)) return rc;
-    if (int rc = e3x_stress(h, o.stress6, o.strain6)) return rc;
-    o.utot0 = a.utot0;
+    const size_t n = a.reqs.size();
+    int lstep[kMaxRhs], ex[kMaxRhs], rs[kMaxRhs];
+    double gg[kMaxRhs], ut[kMaxRhs], g2[kMaxRhs], st[6 * kMaxRhs], sn[6 * kMaxRhs];
+    if (int rc = e3x_dembx_end(h, lstep, gg, nullptr, ex, rs)) return rc;
+    if (int rc = e3x_energy(h, ut, g2)) return rc;
+    if (int rc = e3x_stress(h, st, sn)) return rc;
+    for (size_t k = 0; k < n; k++) {
+        const e3x_request &q = a.reqs[k].rq;
+        if (q.stress_field || q.vm)  // the optional full-field response (f90:2639-2833)
+            if (int rc = e3x_stress_fullfield(h, (int)k, q.stress_field, q.vm)) return rc;
+    }
+    for (size_t k = 0; k < n; k++) {
+        e3x_response &o = out[k];
+        o.lstep = lstep[k];
+        o.gg = gg[k];
+        o.exit_flag = ex[k];
+        o.reset_count = rs[k];
+        o.utot = ut[k];
+        o.utot0 = a.utot0[k];
+        for (int q = 0; q < 6; q++) {
+            o.stress6[q] = st[6 * k + q];
+            o.strain6[q] = sn[6 * k + q];
+        }
+    }
     return E3X_OK;
+}
+
+void respond(Server &S, const Active &a, std::vector<e3x_response> &out, int status)
+{
+    for (e3x_response &o : out) {
+        if (status != E3X_OK) o.status = status;
+        o.t_admit = a.t_admit;
+        o.t_done = S.now();
+        S.done.push_back(o);
+    }
 }
 
 int progress(Server &S, bool *changed)
@@ -222,25 +305,44 @@ int progress(Server &S, bool *changed)
             ++i;
             continue;
         }
-        e3x_response o = make_response(a.req, rc);
-        if (rc == E3X_OK) o.status = finish_solve(a, o);
+        std::vector<e3x_response> out;
+        for (const Request &r : a.reqs) out.push_back(make_response(r, E3X_OK));
+        if (rc == E3X_OK) rc = finish_solve(a, out);
         else (void)e3x_dembx_end(a.hd.h, nullptr, nullptr, nullptr, nullptr, nullptr);
-        o.t_admit = a.t_admit;
-        o.t_done = S.now();
-        S.done.push_back(o);
-        if (o.status == E3X_OK) retire_handle(S, a.hd);
+        respond(S, a, out, rc);
+        if (rc == E3X_OK) retire_handle(S, a.hd);
         else drop_handle(S, a.hd);
         S.active.erase(S.active.begin() + (std::ptrdiff_t)i);
         *changed = true;
     }
-    // admit queued requests: first fit in arrival order
+    // admit queued requests: first fit in arrival order; a request of a small grid takes
+    // the queued requests it can share a batch with (in arrival order) along
     for (auto it = S.queue.begin(); it != S.queue.end() && (int)S.active.size() < S.max_active;) {
+        if (!labels_ok(*it)) {
+            e3x_response o = make_response(*it, report_error(E3X_EPHASE, "phase label in pix out of bounds"));
+            o.t_admit = o.t_done = S.now();
+            S.done.push_back(o);
+            it = S.queue.erase(it);
+            *changed = true;
+            continue;
+        }
         const e3x_request &q = it->rq;
-        const int64_t need = footprint(q.nx, q.ny, q.nz, std::max(kPoolPhases, q.nphase),
-                                       q.precision == E3X_PREC_MIXED, q.ldemb);
+        std::vector<std::list<Request>::iterator> group{it};
+        if (S.batch_members > 1 && (int64_t)q.nx * q.ny * q.nz <= S.batch_max_nodes)
+            for (auto jt = std::next(it); jt != S.queue.end() && (int)group.size() < S.batch_members; ++jt)
+                if (same_batch(jt->rq, q) && labels_ok(*jt)) group.push_back(jt);
+        const int members = group.size() > 1 ? (int)group.size() : 0;
+        int np_each = 0;
+        for (auto &g : group) np_each = std::max(np_each, g->rq.nphase);
+        bool fields = false;  // the full-field staging a handle allocates on first use
+        for (auto &g : group) fields |= g->rq.stress_field != nullptr || g->rq.vm != nullptr;
+        const int64_t need = footprint(q.nx, q.ny, q.nz, std::max(kPoolPhases, np_each),
+                                       q.precision == E3X_PREC_MIXED, q.ldemb) *
+                                 std::max(members, 1) +
+                             (fields ? 7 * 8 * (int64_t)q.nx * q.ny * q.nz : 0);
         Handle hd;
         bool fits = true;
-        if (int rc = acquire_handle(S, q, need, &hd, &fits)) {
+        if (int rc = acquire_handle(S, q, members, np_each, fields, need, &hd, &fits)) {
             e3x_response o = make_response(*it, rc);
             o.t_admit = o.t_done = S.now();
             S.done.push_back(o);
@@ -253,19 +355,19 @@ int progress(Server &S, bool *changed)
             continue;
         }
         Active a;
-        a.req = std::move(*it);
         a.hd = hd;
         a.t_admit = S.now();
+        for (auto &g : group) a.reqs.push_back(std::move(*g));
+        for (size_t k = 1; k < group.size(); k++) S.queue.erase(group[k]);  // never the current one
         it = S.queue.erase(it);
         *changed = true;
         if (int rc = start_solve(a)) {
-            e3x_response o = make_response(a.req, rc);
-            o.t_admit = a.t_admit;
-            o.t_done = S.now();
-            S.done.push_back(o);
+            std::vector<e3x_response> out;
+            for (const Request &r : a.reqs) out.push_back(make_response(r, rc));
+            respond(S, a, out, rc);
             if (reinterpret_cast<Solver *>(a.hd.h)->run.on)
                 (void)e3x_dembx_end(a.hd.h, nullptr, nullptr, nullptr, nullptr, nullptr);
-            // an input error (a bad label, say) leaves the handle sound; a device error may not
+            // an input error leaves the handle sound; a device error may not
             if (rc == E3X_EPHASE || rc == E3X_EINVAL) retire_handle(S, a.hd);
             else drop_handle(S, a.hd);
             continue;
@@ -386,6 +488,17 @@ int e3x_server_drain(e3x_server *sv)
         if (!changed && S.active.empty() && !S.queue.empty())
             return report_error(E3X_ENOMEM, "queued requests cannot be admitted under the memory budget");
     }
+    return E3X_OK;
+}
+
+int e3x_server_set_batching(e3x_server *sv, int max_members, int64_t max_nodes)
+{
+    if (!sv) return report_error(E3X_EINVAL, "NULL server");
+    if (max_members < 1 || max_members > kMaxRhs) return report_error(E3X_EINVAL, "max_members must be in [1, 6]");
+    if (max_nodes < 0) return report_error(E3X_EINVAL, "max_nodes must be >= 0");
+    Server &S = SV(sv);
+    S.batch_members = max_members;
+    S.batch_max_nodes = max_nodes;
     return E3X_OK;
 }
 

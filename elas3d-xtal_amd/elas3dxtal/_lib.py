@@ -80,7 +80,8 @@ class Request(C.Structure):
                 ("pix", _i32p), ("orientation", _dp),
                 ("c11", C.c_double), ("c12", C.c_double), ("c44", C.c_double),
                 ("phasemod", C.c_double * 4), ("flag_m", C.c_int), ("strain6", C.c_double * 6),
-                ("precision", C.c_int), ("replace_every", C.c_int), ("tol", C.c_float), ("ldemb", C.c_int)]
+                ("precision", C.c_int), ("replace_every", C.c_int), ("tol", C.c_float), ("ldemb", C.c_int),
+                ("stress_field", _dp), ("vm", _dp)]
 
 
 class Response(C.Structure):
@@ -100,6 +101,7 @@ SERVE_SIGNATURES = {
     "e3x_server_progress": (C.c_int, [_vp, _ip, _ip, _ip]),
     "e3x_server_take": (C.c_int, [_vp, C.POINTER(Response), _ip]),
     "e3x_server_drain": (C.c_int, [_vp]),
+    "e3x_server_set_batching": (C.c_int, [_vp, C.c_int, C.c_int64]),
     "e3x_server_clock": (C.c_int, [_vp, _dp]),
     "e3x_server_memory": (C.c_int, [_vp, C.POINTER(C.c_int64), _ip]),
 }

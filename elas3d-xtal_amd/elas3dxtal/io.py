@@ -1,6 +1,7 @@
 """Bindings of libe3x_io.so (include/e3x_io.h): HDF5 in the reference's schema and the
 seeded periodic microstructure generator."""
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -30,6 +31,9 @@ SIGNATURES = {
     "e3x_generate": (C.c_int, [C.POINTER(GenParams), _i32p, _dp, _dp, _dp]),
     "e3x_rodrigues_from_bunge": (C.c_int, [C.c_int, _dp, _dp]),
     "e3x_bunge_from_rodrigues": (C.c_int, [C.c_int, _dp, _dp]),
+    "e3x_request_from_files": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(L.Request), C.POINTER(_i32p),
+                                         C.POINTER(_dp), C.c_char_p, C.c_int]),
+    "e3x_request_free": (None, [_i32p, _dp]),
 }
 _io = None
 
@@ -149,3 +153,20 @@ def format_top_peaks(shape, index, value, xvec=None, yvec=None, zvec=None, nsave
 def write_top_peaks(path, shape, index, value, xvec=None, yvec=None, zvec=None):
     with open(path, "w") as f:
         f.write(format_top_peaks(shape, index, value, xvec, yvec, zvec))
+
+
+def request_from_files(h5_path, controls=""):
+    """A serving request from an input file in the reference's schema and JSON controls
+    (e3x_request_from_files).  Returns (Request, pix pointer, orientation pointer,
+    full-field path or None); free the pointers with request_free once answered."""
+    rq = L.Request()
+    pix, ori = _i32p(), _dp()
+    path = C.create_string_buffer(4096)
+    text = controls if isinstance(controls, str) else json.dumps(controls)
+    _check(load().e3x_request_from_files(h5_path.encode(), text.encode(), C.byref(rq), C.byref(pix), C.byref(ori),
+                                         path, 4096))
+    return rq, pix, ori, (path.value.decode() or None)
+
+
+def request_free(pix, ori):
+    load().e3x_request_free(pix, ori)

@@ -11,7 +11,8 @@ of different grid sizes on one GPU, admitted by device memory.
 
 Each admitted request runs the reference's main sequence (ppixel, femat, affine guess,
 energy, dembx, energy, stress: src/elas3dxtal_pcg.f90:380-519) on a device-resident
-handle; its results do not depend on what else is running.
+handle, alone or, for small grids, as a member of a batch with other queued requests on
+the same grid; its results do not depend on what else is running.
 """
 import ctypes as C
 
@@ -30,11 +31,21 @@ def _response_dict(r):
 
 
 class Server:
-    def __init__(self, device=0, hbm_budget=0, max_active=8):
+    def __init__(self, device=0, hbm_budget=0, max_active=8, batch_members=None, batch_max_nodes=None):
         self._lib = L.load()
         h = C.c_void_p()
         L.check(self._lib.e3x_server_create(device, int(hbm_budget), int(max_active), C.byref(h)))
         self._h = h
+        self._files = {}   # id -> arrays and full-field buffers of requests from files
+        if batch_members is not None or batch_max_nodes is not None:
+            self.set_batching(6 if batch_members is None else batch_members,
+                              128 ** 3 if batch_max_nodes is None else batch_max_nodes)
+
+    def set_batching(self, max_members, max_nodes):
+        """Co-batching of small grids (e3x_server_set_batching): up to max_members queued
+        requests on one grid of at most max_nodes nodes (same precision and PCG controls)
+        share one batch handle; max_members = 1 turns it off."""
+        L.check(self._lib.e3x_server_set_batching(self._h, int(max_members), int(max_nodes)))
 
     def close(self):
         if self._h is not None and self._h.value:
@@ -82,6 +93,38 @@ class Server:
         L.check(self._lib.e3x_server_submit(self._h, C.byref(rq), C.byref(rid)))
         return rid.value
 
+    def submit_files(self, h5_path, controls=""):
+        """Queue a request read from an input file in the reference's schema with JSON
+        controls (a str, or a dict to encode; keys in include/e3x_io.h,
+        e3x_request_from_files).  With a "fullfield" path the response comes with the
+        full field, written there as fullfield_poly.h5 (the reference's output schema)
+        when take() returns it."""
+        from . import io
+        rq, pix, ori, ff = io.request_from_files(h5_path, controls)
+        ns = rq.nx * rq.ny * rq.nz
+        keep = dict(pix=pix, ori=ori, path=ff, ns=ns)
+        if ff:
+            keep["stress"], keep["vm"] = np.zeros(6 * ns), np.zeros(ns)
+            rq.stress_field = L.dptr(keep["stress"])
+            rq.vm = L.dptr(keep["vm"])
+        rid = C.c_int64()
+        try:
+            L.check(self._lib.e3x_server_submit(self._h, C.byref(rq), C.byref(rid)))
+        except Exception:
+            io.request_free(pix, ori)
+            raise
+        self._files[rid.value] = keep
+        return rid.value
+
+    def _finish_file(self, out):
+        from . import io
+        keep = self._files.pop(out["id"])
+        if keep["path"] and out["status"] == L.E3X_OK:
+            pix = np.ctypeslib.as_array(keep["pix"], shape=(keep["ns"],)).copy()
+            io.write_fullfield(keep["path"], pix, keep["vm"], keep["stress"].reshape(6, -1))
+            out["fullfield"] = keep["path"]
+        io.request_free(keep["pix"], keep["ori"])
+
     def progress(self):
         """One scheduling pass; returns (active, queued, finished-not-taken)."""
         a, q, d = C.c_int(), C.c_int(), C.c_int()
@@ -96,7 +139,10 @@ class Server:
             L.check(self._lib.e3x_server_take(self._h, C.byref(r), C.byref(got)))
             if not got.value:
                 return out
-            out.append(_response_dict(r))
+            d = _response_dict(r)
+            if d["id"] in self._files:
+                self._finish_file(d)
+            out.append(d)
 
     def drain(self):
         L.check(self._lib.e3x_server_drain(self._h))

@@ -57,6 +57,33 @@ int e3x_h5_write_fullfield(const char *path, int64_t ns, const int32_t *pix, con
 /* Read the three datasets of a full-field file back (any pointer may be NULL). */
 int e3x_h5_read_fullfield(const char *path, int64_t ns, int32_t *pix, double *vm, double *stress);
 
+/* ------------------------------------------------------------- serving requests */
+
+struct e3x_request;  /* include/e3x_serve.h */
+
+/*
+ * A serving request (e3x_serve.h) from files; no counterpart in the reference, whose runs
+ * take their parameters from the source.  The microstructure comes from an input file in
+ * the reference's schema (as e3x_h5_read_input: the grid from the /pix "dimensions"
+ * attribute or the "grid" control, nphase = number of /orientation triples + 1) and the
+ * controls from one flat JSON object, every key optional, each default the reference's:
+ *   "grid": [nx, ny, nz]            "c11", "c12", "c44": Pa (246.5e9, 147.3e9, 124.7e9)
+ *   "zener": A (c11 = c12 + 2 c44 / A; not with "c11")
+ *   "strain6": [exx, eyy, ezz, exz, eyz, exy] tensor components (float32(1e-3), 0, ...)
+ *   "phasemod": [K, G] of the pore phase (or [K1, G1, K2, G2] for flag_m 0), "flag_m": 1
+ *   "precision": "fp64" | "mixed", "replace_every": 50, "tol": float32(1e-8),
+ *   "ldemb": 10000, "fullfield": path of the full-field response (fullfield_poly.h5)
+ * Unknown keys are errors (E3X_EINVAL).  Fills *rq (its stress_field and vm NULL) and
+ * returns the arrays it points to in *pix and *orientation (free them with
+ * e3x_request_free once the request is answered), and the "fullfield" path in
+ * fullfield_path[path_cap] ("" when absent): the caller then sets rq->stress_field and
+ * rq->vm to buffers of 6*ns and ns doubles and, with the response, writes the file with
+ * e3x_h5_write_fullfield.
+ */
+int e3x_request_from_files(const char *h5_path, const char *json, struct e3x_request *rq, int32_t **pix,
+                           double **orientation, char *fullfield_path, int path_cap);
+void e3x_request_free(int32_t *pix, double *orientation);
+
 /* ------------------------------------------------------------------------ generator */
 
 enum e3x_pore_mode {

@@ -7,8 +7,9 @@
  * the solves that have finished, admits queued requests while the active count stays
  * under `max_active` and the device memory of the server's solver handles under
  * `hbm_budget_bytes` (first fit in arrival order: a request that does not fit yet lets
- * later, smaller ones pass), and keeps the PCG of every active solve running on its
- * own stream.  Solves of different grid sizes therefore run side by side on the GPU,
+ * later, smaller ones pass; requests on one small grid are admitted together as a batch,
+ * e3x_server_set_batching), and keeps the PCG of every active solve running on its own
+ * stream.  Solves of different grid sizes therefore run side by side on the GPU,
  * and a request that arrives while others run starts as soon as a slot and memory are
  * free.  Each admitted request runs the main program's sequence (f90:380-519: ppixel,
  * femat, affine guess, energy, dembx, energy, stress) with its state resident on the
@@ -43,6 +44,11 @@ typedef struct e3x_request {
     int replace_every;         /* mixed form: true-residual replacement period (0: never)       */
     float tol;                 /* stop tolerance, float32(1e-8) in the reference                */
     int ldemb;                 /* iteration limit                                               */
+    double *stress_field;      /* optional full-field response (may be NULL): host buffers the */
+    double *vm;                /* server fills before the response is ready, stress (6,ns) as  */
+                               /* Fortran stress_field(ns,6) and von Mises (ns), as            */
+                               /* stress_fullfield_OpenMP computes them (f90:2639-2833); owned */
+                               /* by the caller until the response is taken                    */
 } e3x_request;
 
 typedef struct e3x_response {
@@ -76,6 +82,16 @@ int e3x_server_take(e3x_server *sv, e3x_response *out, int *got);
 
 /* Run progress until nothing is queued or active (waits on the GPU between passes). */
 int e3x_server_drain(e3x_server *sv);
+
+/*
+ * Co-batching: queued requests on one grid of at most max_nodes nodes, with the same
+ * precision and PCG controls (replace_every, tol, ldemb, flag_m), are admitted together,
+ * up to max_members of them (1..6; 1 turns co-batching off), as the members of one batch
+ * handle (e3x_create_batch): their problems share every launch.  A batch takes one slot of
+ * max_active and its requests finish together; each result is bit for bit the one its
+ * request gets alone.  Defaults: 6 members, 128^3 nodes.
+ */
+int e3x_server_set_batching(e3x_server *sv, int max_members, int64_t max_nodes);
 
 /* Seconds since the server was created (the clock of t_submit, t_admit, t_done). */
 int e3x_server_clock(const e3x_server *sv, double *seconds);

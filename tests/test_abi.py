@@ -36,7 +36,7 @@ def test_serving_library_exports_header_symbols():
 
 def test_request_response_layout_matches_header():
     """The ctypes structs have the C layout (offsets of the last members)."""
-    assert ctypes.sizeof(L.Request) == 4 * 4 + 2 * 8 + 3 * 8 + 4 * 8 + 4 + 4 + 6 * 8 + 4 + 4 + 4 + 4
+    assert ctypes.sizeof(L.Request) == 4 * 4 + 2 * 8 + 3 * 8 + 4 * 8 + 4 + 4 + 6 * 8 + 4 + 4 + 4 + 4 + 2 * 8
     assert L.Response.t_done.offset == ctypes.sizeof(L.Response) - 8
 
 

@@ -1,6 +1,6 @@
 """The request loop (include/e3x_serve.h) on the GPU: every result is the same served
-in a mix of concurrent solves as alone, under memory-based admission, with requests
-arriving while others run."""
+in a mix of concurrent solves (and co-batched with other requests on its grid) as alone,
+under memory-based admission, with requests arriving while others run."""
 import numpy as np
 import pytest
 
@@ -63,6 +63,47 @@ def test_served_results_equal_solves_alone():
         assert got[i]["t_submit"] <= got[i]["t_admit"] <= got[i]["t_done"]
 
 
+def test_cobatched_requests_equal_solves_alone():
+    """Requests on the same small grids queued together are admitted as batches (two grid
+    sizes, both precisions, some with a tighter tolerance, so they batch in groups);
+    every result is bit for bit its solve alone."""
+    qs = []
+    for k, q in enumerate(_requests(14, seed=15)):
+        shape = (12, 10, 8) if k % 2 else (16, 16, 8)
+        q = dict(q, shape=shape)
+        q["pix"], q["rod"] = voronoi_case(shape, q["nphase"] - 1, seed=1500 + k, pore_frac=0.02)
+        q["prec"] = L.PREC_MIXED if k % 3 else L.PREC_FP64
+        qs.append(q)
+    with Server(max_active=3, batch_members=4) as sv:
+        ids = [_submit(sv, q) for q in qs]     # all queued before the first progress pass
+        sv.drain()
+        got = {r["id"]: r for r in sv.take()}
+    assert sorted(got) == ids
+    # admissions together: the batches' members share their admission time
+    admits = [got[i]["t_admit"] for i in ids]
+    assert len(set(admits)) < len(admits)
+    for i, q in zip(ids, qs):
+        _check_same(got[i], _alone(q))
+
+
+def test_batching_off_gives_the_same_results():
+    qs = [dict(q, shape=(12, 12, 12)) for q in _requests(4, seed=16)]
+    for k, q in enumerate(qs):
+        q["pix"], q["rod"] = voronoi_case(q["shape"], q["nphase"] - 1, seed=1600 + k, pore_frac=0.02)
+        q["prec"] = L.PREC_MIXED
+    res = []
+    for members in (1, 4):
+        with Server(max_active=2, batch_members=members) as sv:
+            ids = [_submit(sv, q) for q in qs]
+            sv.drain()
+            got = {r["id"]: r for r in sv.take()}
+        res.append([got[i] for i in ids])
+    for a, b in zip(*res):
+        assert a["lstep"] == b["lstep"] and a["utot"] == b["utot"]
+        assert np.array_equal(a["stress6"], b["stress6"])
+    assert len({r["t_admit"] for r in res[1]}) == 1      # one batch of four
+
+
 def _handle_bytes(shape, mixed, ldemb=10000, phases=1024):
     """Device bytes the server books for a one-rhs handle (its admission estimate)."""
     ns = int(np.prod(shape))
@@ -83,7 +124,7 @@ def test_admission_by_memory_and_arrivals_during_work():
     for k, q in enumerate(qs):
         q["pix"], q["rod"] = voronoi_case(q["shape"], q["nphase"] - 1, seed=1200 + k, pore_frac=0.02)
     budget = int(2.5 * _handle_bytes((12, 12, 12), mixed=True))
-    with Server(hbm_budget=budget, max_active=6) as sv:
+    with Server(hbm_budget=budget, max_active=6, batch_members=1) as sv:   # one handle per request
         ids = [_submit(sv, q) for q in qs[:3]]
         active, queued, _ = sv.progress()
         assert (active, queued) == (2, 1)
@@ -111,3 +152,42 @@ def test_bad_request_is_reported_not_fatal():
         got = {r["id"]: r for r in sv.take()}
     assert got[i_bad]["status"] == L.E3X_EPHASE
     _check_same(got[i_ok], _alone(qs[1]))
+
+
+def test_requests_from_files_with_full_field_response(tmp_path):
+    """The serving front-end on files: requests read from inputs in the reference's schema
+    with JSON controls, one asking for the full field.  Each answer equals its solve alone;
+    the full-field file has the reference's output datasets (f90:2840-2874) and holds that
+    solve's stress and von Mises fields bit for bit."""
+    from elas3dxtal import io
+    qs = []
+    for k, shape in enumerate([(12, 10, 8), (12, 10, 8), (10, 10, 10)]):
+        ng = 4 + k
+        pix, rod = voronoi_case(shape, ng, seed=1700 + k, pore_frac=0.02)
+        path = str(tmp_path / f"in{k}.h5")
+        io.write_input(path, shape, pix, rod)
+        ctl = {"zener": 1.5 + k, "strain6": [0, 0, 0, 0, 0.5e-3, 0] if k == 1 else [1e-3, 0, 0, 0, 0, 0],
+               "precision": "mixed" if k == 2 else "fp64"}
+        if k == 0:
+            ctl["fullfield"] = str(tmp_path / "fullfield_poly.h5")
+        qs.append(dict(pix=pix, rod=rod, shape=shape, nphase=ng + 1, c11=C12 + 2 * C44 / ctl["zener"],
+                       e6=np.array(ctl["strain6"]), prec=L.PREC_MIXED if k == 2 else L.PREC_FP64, path=path, ctl=ctl))
+    with Server(max_active=2) as sv:
+        ids = [sv.submit_files(q["path"], q["ctl"]) for q in qs]
+        sv.drain()
+        got = {r["id"]: r for r in sv.take()}
+    assert sorted(got) == ids
+    for i, q in zip(ids, qs):
+        _check_same(got[i], _alone(q))
+    assert got[ids[0]]["fullfield"] == qs[0]["ctl"]["fullfield"]
+    assert "fullfield" not in got[ids[1]]
+    q = qs[0]
+    ns = int(np.prod(q["shape"]))
+    pix2, vm, st = io.read_fullfield(q["ctl"]["fullfield"], ns)
+    r, s = solve(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], C12, C44, q["e6"], keep=True)
+    try:
+        st1, vm1 = s.stress_fullfield(0)
+    finally:
+        s.close()
+    assert np.array_equal(pix2, q["pix"])
+    assert np.array_equal(vm, vm1) and np.array_equal(st, st1)

@@ -7,6 +7,7 @@ from scipy.spatial import cKDTree
 from scipy.spatial.transform import Rotation
 
 import _oracle as oc
+from elas3dxtal import _lib as L
 from elas3dxtal import configs, io
 
 
@@ -169,3 +170,40 @@ def test_config_materials():
         assert abs(configs.c11_for_zener(a) - want) < 1e-9
     e = configs.six_unit_strains()
     assert e[0, 0] == np.float32(1e-3) and e[5, 5] == 0.5 * np.float32(1e-3)
+
+
+def test_request_from_files_reads_schema_and_controls(tmp_path):
+    """A serving request from the reference's input file and JSON controls: the grid from
+    the /pix dimensions attribute, nphase from /orientation, the reference's defaults for
+    what the controls leave out, and errors for unknown or conflicting keys."""
+    from _cases import voronoi_case
+    pix, rod = voronoi_case((8, 7, 6), 4, seed=1)
+    p = str(tmp_path / "in.h5")
+    io.write_input(p, (8, 7, 6), pix, rod)
+    rq, pp, oo, ff = io.request_from_files(p, "")
+    try:
+        assert (rq.nx, rq.ny, rq.nz, rq.nphase) == (8, 7, 6, 5)
+        assert (rq.c11, rq.c12, rq.c44) == (246.5e9, 147.3e9, 124.7e9)
+        assert list(rq.strain6) == [float(np.float32(1e-3)), 0, 0, 0, 0, 0]
+        assert rq.tol == float(np.float32(1e-8)) and rq.ldemb == 10000 and rq.precision == 0
+        assert rq.flag_m == 1 and ff is None
+        assert np.array_equal(np.ctypeslib.as_array(pp, shape=(pix.size,)), pix)
+        assert np.array_equal(np.ctypeslib.as_array(oo, shape=(rod.size,)), rod.ravel())
+    finally:
+        io.request_free(pp, oo)
+    ctl = {"zener": 2.0, "precision": "mixed", "replace_every": 25, "strain6": [0, 0, 1e-3, 0, 0, 0],
+           "fullfield": str(tmp_path / "ff.h5")}
+    rq, pp, oo, ff = io.request_from_files(p, ctl)
+    io.request_free(pp, oo)
+    assert rq.c11 == 147.3e9 + 2 * 124.7e9 / 2.0 and rq.precision == 1 and rq.replace_every == 25
+    assert list(rq.strain6) == [0, 0, 1e-3, 0, 0, 0] and ff == ctl["fullfield"]
+    for bad in ('{"bogus": 1}', '{"c11": 1e9, "zener": 2}', '{"strain6": [1, 2]}', '{"precision": "fp16"}',
+                '{"c11": 1e9', '[1, 2]'):
+        with pytest.raises(L.E3xError) as e:
+            io.request_from_files(p, bad)
+        assert e.value.code == L.E3X_EINVAL, bad
+    with pytest.raises(L.E3xError) as e:        # the grid is neither in the file nor given
+        q = str(tmp_path / "noattr.h5")
+        io.write_input(q, (8, 7, 6), pix, rod)
+        rq, pp, oo, _ = io.request_from_files(q, '{"grid": [8, 7, 5]}')   # wrong size: f90:2939-2942
+    assert e.value.code == L.E3X_EINVAL
