@@ -99,13 +99,15 @@ def run_gpu(w, c11, c12, c44, e6, steps, warmup, precision):
             apply_n += n
         s.set_apply_timing(False)
         replay_ms = s.time_apply(precision=precision, reps=20)
+        tuned = s.tuning(precision)
         _, vm = s.stress_fullfield()
         u = s.get(L.FIELD_U)
         s.affine_guess()
         u_aff = s.get(L.FIELD_U)
         return dict(u_affine=u_aff, walls=walls, pcg_ms=pcg_ms, lsteps=lsteps, exit=exit_flag, utot=float(ut[0]),
                     stress6=st[0].copy(), strain6=sn[0].copy(), u=u, vm=vm,
-                    apply_ms=sum(apply_ms) / max(apply_n, 1), apply_launches=apply_n, replay_ms=replay_ms)
+                    apply_ms=sum(apply_ms) / max(apply_n, 1), apply_launches=apply_n, replay_ms=replay_ms,
+                    tuned_waves=tuned[0])
     finally:
         s.close()
 
@@ -332,6 +334,7 @@ def main():
                     launch_time_source=f"HIP events on the solver's stream around the {fp64['apply_launches']} "
                                        "fused applies of the timed PCG loops",
                     replay_avg_launch_us=round(fp64["replay_ms"] * 1e3, 2),
+                    waves_per_simd=fp64["tuned_waves"],
                     traffic_source=("not measured in this run: FETCH_SIZE + WRITE_SIZE per launch from the "
                                     "rocprofv3 --pmc passes summarised in profiles/stencil_apply_cube200_pmc.txt")
                     if traffic else None,
@@ -376,6 +379,7 @@ def main():
             "errors_vs_fp64": output_errors(mx, fp64),
             "apply_us": round(mx["apply_ms"] * 1e3, 2),
             "apply_replay_us": round(mx["replay_ms"] * 1e3, 2),
+            "apply_waves_per_simd": mx["tuned_waves"],
         }
     if not args.no_six:
         six = {}

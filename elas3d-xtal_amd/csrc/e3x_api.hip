@@ -6,7 +6,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -105,6 +108,94 @@ int ensure_hist(Solver &s, int ldemb)
     }
     E3X_HIP(dalloc(s, &s.hist, need));
     s.hist_cap = need;
+    return E3X_OK;
+}
+
+// The launch tuning of the PCG apply, shared by the process's handles of one grid shape,
+// right-hand-side layout and precision.
+struct TuneKey {
+    int nx, ny, nz, nrhs, members, wide, mixed;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(nx, ny, nz, nrhs, members, wide, mixed) <
+               std::tie(o.nx, o.ny, o.nz, o.nrhs, o.members, o.wide, o.mixed);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, double>> g_tuned;
+
+// Mean time of `reps` fused applies as the PCG runs them (the u steps alternating between
+// none and two), the PCG state of every rhs set to run with lambda = gamma = 0 (u and z
+// keep their values, h = z), then restored.  Overwrites h_alt / h32_alt, Ah and partials.
+int time_fused_apply(Solver &s, bool mixed, int reps, double *avg_ms)
+{
+    PcgState saved[kMaxRhs];
+    E3X_HIP(hipMemcpyAsync(s.st_host, s.st, sizeof(PcgState) * s.nrhs, hipMemcpyDeviceToHost, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    for (int r = 0; r < s.nrhs; r++) {
+        saved[r] = s.st_host[r];
+        s.st_host[r].stop_iter = 0;
+        s.st_host[r].lambda = 0.0;  // u stays as it is
+        s.st_host[r].lambda_prev = 0.0;
+        s.st_host[r].gamma = 0.0;   // h = z
+        s.st_host[r].reset = 0;
+    }
+    E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(launch_apply_only(s, mixed, 2));  // warm-up
+    E3X_HIP(hipEventRecord(s.ev[0], s.stream));
+    for (int i = 0; i < reps; i++) E3X_HIP(launch_apply_only(s, mixed, (i & 1) ? 2 : 0));
+    E3X_HIP(hipEventRecord(s.ev[1], s.stream));
+    E3X_HIP(hipEventSynchronize(s.ev[1]));
+    float ms = 0;
+    E3X_HIP(hipEventElapsedTime(&ms, s.ev[0], s.ev[1]));
+    *avg_ms = ms / reps;
+    for (int r = 0; r < s.nrhs; r++) s.st_host[r] = saved[r];
+    E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(hipStreamSynchronize(s.stream));
+    return E3X_OK;
+}
+
+// Choose the waves per SIMD of the PCG apply in this precision: forced (e3x_set_tuning),
+// already known for the shape, or measured now (each variant, a few launches on the
+// handle's data; the fastest kept).  Every variant computes the same bits.
+int tune_apply(Solver &s, bool mixed)
+{
+    const int k = mixed ? 1 : 0;
+    if (s.tuned_waves[k] != 0) return E3X_OK;
+    if (s.tune_force[k] != 0) {
+        s.tuned_waves[k] = s.tune_force[k];
+        return E3X_OK;
+    }
+    const TuneKey key{s.nx, s.ny, s.nz, s.nrhs, s.members, s.wide ? 1 : 0, k};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            s.tuned_waves[k] = it->second.first;
+            s.tuned_apply_ms[k] = it->second.second;
+            return E3X_OK;
+        }
+    }
+    const int *cand = mixed ? kTuneWaves32 : kTuneWaves64;
+    const int ncand = mixed ? 2 : 3;
+    int best = 0;
+    double best_ms = 0;
+    for (int c = 0; c < ncand; c++) {
+        s.tuned_waves[k] = cand[c];
+        double ms = 0;
+        if (int rc = time_fused_apply(s, mixed, 4, &ms)) {
+            s.tuned_waves[k] = 0;
+            return rc;
+        }
+        if (best == 0 || ms < best_ms) {
+            best = cand[c];
+            best_ms = ms;
+        }
+    }
+    s.tuned_waves[k] = best;
+    s.tuned_apply_ms[k] = best_ms;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = {best, best_ms};
     return E3X_OK;
 }
 
@@ -524,6 +615,7 @@ int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const doubl
         if (int rc = ensure_mixed_buffers(s)) return rc;
     if (int rc = ensure_hist(s, ldemb)) return rc;
     const int64_t nv = (int64_t)s.nrhs * 3 * s.ns;
+    if (int rc = tune_apply(s, mixed)) return rc;  // the apply's launch variant (once per shape)
     if (s.time_applies && s.apply_ev.empty()) {
         s.apply_ev.assign(2 * kApplyTimed, nullptr);
         s.apply_it.assign(kApplyTimed, 0);
@@ -942,6 +1034,35 @@ int e3x_last_apply_timing(e3x_solver *h, int *launches, double *avg_ms)
     return E3X_OK;
 }
 
+int e3x_set_tuning(e3x_solver *h, int precision, int waves)
+{
+    if (int rc = check_idle(h)) return rc;
+    Solver &s = S(h);
+    if (precision != E3X_PREC_FP64 && precision != E3X_PREC_MIXED)
+        return fail(E3X_EINVAL, "precision must be E3X_PREC_FP64 or E3X_PREC_MIXED");
+    const int k = precision == E3X_PREC_MIXED ? 1 : 0;
+    bool ok = waves == 0;
+    if (k == 0)
+        for (int w : kTuneWaves64) ok |= w == waves;
+    else
+        for (int w : kTuneWaves32) ok |= w == waves;
+    if (!ok) return fail(E3X_EINVAL, "waves must be 0 or a variant of the precision (FP64 5, 6, 8; FP32 6, 8)");
+    s.tune_force[k] = waves;
+    s.tuned_waves[k] = waves;  // 0: measured again at the next PCG
+    s.tuned_apply_ms[k] = 0;
+    return E3X_OK;
+}
+
+int e3x_get_tuning(e3x_solver *h, int precision, int *waves, double *apply_ms)
+{
+    if (int rc = check_handle(h)) return rc;
+    Solver &s = S(h);
+    const int k = precision == E3X_PREC_MIXED ? 1 : 0;
+    if (waves) *waves = s.tuned_waves[k];
+    if (apply_ms) *apply_ms = s.tuned_apply_ms[k];
+    return E3X_OK;
+}
+
 int e3x_time_apply(e3x_solver *h, int precision, int reps, double *avg_ms)
 {
     if (int rc = check_idle(h)) return rc;
@@ -952,33 +1073,9 @@ int e3x_time_apply(e3x_solver *h, int precision, int reps, double *avg_ms)
     E3X_HIP(hipSetDevice(s.device));
     if (mixed)
         if (int rc = ensure_mixed_buffers(s)) return rc;
-    // save the PCG state, mark every rhs running so the kernel does its work
-    PcgState saved[kMaxRhs];
-    E3X_HIP(hipMemcpyAsync(s.st_host, s.st, sizeof(PcgState) * s.nrhs, hipMemcpyDeviceToHost, s.stream));
-    E3X_HIP(hipStreamSynchronize(s.stream));
-    for (int r = 0; r < s.nrhs; r++) {
-        saved[r] = s.st_host[r];
-        s.st_host[r].stop_iter = 0;
-        s.st_host[r].lambda = 0.0;  // u stays as it is
-        s.st_host[r].lambda_prev = 0.0;
-        s.st_host[r].gamma = 0.0;   // h = z
-        s.st_host[r].reset = 0;
-    }
-    E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
+    if (int rc = tune_apply(s, mixed)) return rc;
     if (mixed) E3X_HIP(launch_to_f32(s.h, s.h32, (int64_t)s.nrhs * 3 * s.ns, s.stream));
-    // the PCG's applies alternate between none and two u steps: time them in turn
-    E3X_HIP(launch_apply_only(s, mixed, 2));  // warm-up
-    E3X_HIP(hipEventRecord(s.ev[0], s.stream));
-    for (int i = 0; i < reps; i++) E3X_HIP(launch_apply_only(s, mixed, (i & 1) ? 2 : 0));
-    E3X_HIP(hipEventRecord(s.ev[1], s.stream));
-    E3X_HIP(hipEventSynchronize(s.ev[1]));
-    float ms = 0;
-    E3X_HIP(hipEventElapsedTime(&ms, s.ev[0], s.ev[1]));
-    *avg_ms = ms / reps;
-    for (int r = 0; r < s.nrhs; r++) s.st_host[r] = saved[r];
-    E3X_HIP(hipMemcpyAsync(s.st, s.st_host, sizeof(PcgState) * s.nrhs, hipMemcpyHostToDevice, s.stream));
-    E3X_HIP(hipStreamSynchronize(s.stream));
-    return E3X_OK;
+    return time_fused_apply(s, mixed, reps, avg_ms);
 }
 
 }  // extern "C"

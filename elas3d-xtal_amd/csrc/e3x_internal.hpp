@@ -169,6 +169,11 @@ struct Solver {
     int apply_n = 0;                   // pairs recorded in the PCG in progress
     int last_apply_n = 0;
     double last_apply_ms = 0;          // mean over the launches of the last PCG
+    // launch tuning of the PCG's stencil apply, [FP64, FP32]: waves per SIMD its register
+    // budget is compiled for (0: not chosen yet; the tuner picks per grid shape, e3x_tuning)
+    int tuned_waves[2] = {0, 0};
+    int tune_force[2] = {0, 0};        // e3x_set_tuning: 0 measure, else the waves to use
+    double tuned_apply_ms[2] = {0, 0};
     bool have_pix = false, have_femat = false, have_precond = false, have_precond32 = false;
     double last_precond_ms = 0, last_pcg_ms = 0;
     int64_t bytes = 0;
@@ -190,6 +195,8 @@ hipError_t launch_apply_plain(Solver &s, const double *x, double *y);  // y = K 
 // 2 of u_steps_of_apply), then hAh and lambda
 hipError_t launch_apply_hah(Solver &s, bool mixed, int it, int usteps);
 hipError_t launch_apply_only(Solver &s, bool mixed, int usteps);  // fused apply with the current state
+// the waves-per-SIMD variants of the PCG apply: FP64 {5, 6, 8}, FP32 {6, 8}
+constexpr int kTuneWaves64[3] = {5, 6, 8}, kTuneWaves32[2] = {6, 8};
 hipError_t launch_resid(Solver &s);                           // g32 = K u + b in FP64
 hipError_t launch_tile_order(Solver &s, int member = 0);      // perm from pix (of a batch member)
 // e3x_pcg.hip

@@ -606,13 +606,30 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
 // sharing the phase staging, classification and coefficient loads, ran 2-4 % slower at
 // 200^3: the per-rhs FMA, LDS and staging work dominates, and the second halo costs
 // occupancy.)
+template <typename T, int MODE, int W>
+void launch_stencil_w(const Solver &s, const StencilArgs<T> &A, dim3 grid)
+{
+    if (s.wide) hipLaunchKernelGGL((k_stencil<T, MODE, W, uint32_t>), grid, dim3(kSB), 0, s.stream, A);
+    else hipLaunchKernelGGL((k_stencil<T, MODE, W, uint16_t>), grid, dim3(kSB), 0, s.stream, A);
+}
+
+// The PCG applies (kApply, kApplyFused) run at the occupancy the handle's tuning chose
+// (tuned_waves: the waves per SIMD the compiler is held to, so their register budget);
+// the other modes at the defaults, six waves for the FP64 body and eight for FP32.
 template <typename T, int MODE>
 void dispatch_stencil(const Solver &s, const StencilArgs<T> &A, int rows = -1)
 {
-    const dim3 grid((unsigned)stencil_blocks(s), (unsigned)(rows < 0 ? s.nrhs : rows)), block(kSB);
-    constexpr int W = sizeof(T) == 4 ? 8 : 6;
-    if (s.wide) hipLaunchKernelGGL((k_stencil<T, MODE, W, uint32_t>), grid, block, 0, s.stream, A);
-    else hipLaunchKernelGGL((k_stencil<T, MODE, W, uint16_t>), grid, block, 0, s.stream, A);
+    const dim3 grid((unsigned)stencil_blocks(s), (unsigned)(rows < 0 ? s.nrhs : rows));
+    constexpr bool PCG = MODE == kApply || MODE == kApplyFused;
+    const int w = PCG ? s.tuned_waves[sizeof(T) == 4 ? 1 : 0] : 0;
+    if constexpr (sizeof(T) == 4) {
+        if (PCG && w == 6) launch_stencil_w<T, MODE, 6>(s, A, grid);
+        else launch_stencil_w<T, MODE, 8>(s, A, grid);
+    } else {
+        if (PCG && w == 5) launch_stencil_w<T, MODE, 5>(s, A, grid);
+        else if (PCG && w == 8) launch_stencil_w<T, MODE, 8>(s, A, grid);
+        else launch_stencil_w<T, MODE, 6>(s, A, grid);
+    }
 }
 
 }  // namespace

@@ -69,6 +69,8 @@ SIGNATURES = {
     "e3x_time_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "e3x_stress_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _dp, _vp, _vp, _vp]),
     "e3x_set_apply_timing": (C.c_int, [_vp, C.c_int]),
+    "e3x_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "e3x_get_tuning": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "e3x_last_apply_timing": (C.c_int, [_vp, _ip, _dp]),
 }
 

@@ -243,6 +243,17 @@ class Elas3D:
         L.check(self._lib.e3x_time_apply(self._h, precision, reps, C.byref(ms)))
         return ms.value
 
+    def set_tuning(self, precision=L.PREC_FP64, waves=0):
+        """Launch variant of the PCG apply (e3x_set_tuning): 0 measures and keeps the
+        fastest at the next PCG; else FP64 5/6/8, FP32 6/8 waves per SIMD."""
+        L.check(self._lib.e3x_set_tuning(self._h, precision, waves))
+
+    def tuning(self, precision=L.PREC_FP64):
+        """-> (waves per SIMD in use, measured mean apply ms)."""
+        w, ms = C.c_int(), C.c_double()
+        L.check(self._lib.e3x_get_tuning(self._h, precision, C.byref(w), C.byref(ms)))
+        return w.value, ms.value
+
     def set_apply_timing(self, on=True):
         """Time the stencil applies of the following PCG runs in the loop itself (HIP
         events on the solver's stream; e3x_set_apply_timing)."""

@@ -329,6 +329,19 @@ int e3x_set_apply_timing(e3x_solver *s, int on);
 int e3x_last_apply_timing(e3x_solver *s, int *launches, double *avg_ms);
 
 /*
+ * Launch tuning of the PCG's stencil apply (no counterpart in the reference): the waves per
+ * SIMD its register budget is compiled for, FP64 {5, 6, 8} and FP32 {6, 8}.  With waves = 0
+ * (the default) the first e3x_dembx* (or e3x_time_apply) of a precision times every
+ * variant on the handle's own data (four launches each) and keeps the fastest; the
+ * choice is then shared by the process's handles of the same grid, right-hand sides and
+ * precision.  waves != 0 forces that variant on this handle.  Every variant computes the
+ * same bits.  e3x_get_tuning returns the variant in use (0 before the first PCG) and its
+ * measured mean apply time in ms (0 when forced or not measured).
+ */
+int e3x_set_tuning(e3x_solver *s, int precision, int waves);
+int e3x_get_tuning(e3x_solver *s, int precision, int *waves, double *apply_ms);
+
+/*
  * Measurement hook (no counterpart in the reference): replay `reps` launches of the
  * PCG's stencil-apply kernel in the form the loop runs from its second iteration on
  * (direction h = z + gamma h_prev, u -= lambda h_prev, Ah = K h with the h.Ah

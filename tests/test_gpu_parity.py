@@ -690,6 +690,32 @@ def test_wide_phase_ids_give_the_same_bits(precision):
     assert np.array_equal(narrow.stress6[0], wide.stress6[0])
 
 
+@pytest.mark.parametrize("precision,variants", [(L.PREC_FP64, (5, 6, 8)), (L.PREC_MIXED, (6, 8))])
+def test_apply_launch_variants_give_the_same_bits(precision, variants):
+    """The tuner's launch variants of the PCG apply (waves per SIMD of its register budget)
+    compute the same bits; with tuning left to the library it picks one of them at the
+    first PCG and reports it."""
+    shape = (24, 20, 16)
+    pix, rod = voronoi_case(shape, 8, seed=4901, pore_frac=0.02)
+    e6 = [oc.AML_F32, 0, 0, 0, 0.3e-3, 0]
+    out = {}
+    for w in variants + (0,):
+        with Elas3D(shape, 9) as s:
+            s.ppixel(pix, rod)
+            s.set_tuning(precision, w)
+            s.femat(C11, C12, C44, e6)
+            s.affine_guess()
+            _, gg0 = s.energy()
+            d = s.dembx(gg0, precision=precision, replace_every=7)
+            ut, _ = s.energy()
+            out[w] = (int(d["lstep"][0]), float(ut[0]), s.get(L.FIELD_U), s.tuning(precision))
+    ref = out[variants[0]]
+    for w, (lstep, ut, u, tun) in out.items():
+        assert lstep == ref[0] and ut == ref[1] and np.array_equal(u, ref[2]), w
+        assert tun[0] == (w if w else tun[0]) and tun[0] in variants
+    assert out[0][3][1] > 0     # measured
+
+
 def test_bad_phase_label_and_call_order():
     with Elas3D((4, 4, 4), 3) as s:
         with pytest.raises(L.E3xError) as ei:
