@@ -156,30 +156,8 @@ def run_six(w, c11, c12, c44, steps, warmup, precision):
 
 
 def serving_requests(n_requests, seed=5000):
-    """The serving stream's requests, each with its own seeded microstructure: grid edge
-    uniform over {64, 96, 128, 160, 200}, 10-800 Voronoi grains with uniform SO(3)
-    orientations, Zener ratio A uniform on [0.5, 5], porosity uniform on [0, 5] % as
-    lognormal spheres (median radius 3 voxels, sigma 0.4), one random unit macro strain
-    of magnitude 1e-3; mixed precision, replacement every 50 iterations.  Generated on the
-    host before the stream starts."""
-    from elas3dxtal import io
-
-    rng = np.random.default_rng(seed)
-    edges = [64, 96, 128, 160, 200]
-    reqs = []
-    for k in range(n_requests):
-        e = int(edges[int(rng.integers(0, len(edges)))])
-        ng = int(rng.integers(10, 801))
-        frac = float(rng.uniform(0.0, 0.05))
-        pix, rod, _, _ = io.generate((e, e, e), ng, 1000 * 5 + k, pore_mode=io.PORES_SPHERES,
-                                     pore_fraction=frac, pore_median=3.0, pore_sigma=0.4)
-        a = float(rng.uniform(0.5, 5.0))
-        e6 = np.zeros(6)
-        j = int(rng.integers(0, 6))
-        e6[j] = 1e-3 if j < 3 else 0.5e-3
-        reqs.append(dict(shape=(e, e, e), nphase=ng + 1, pix=pix, rod=rod,
-                         c11=(configs.C12_GPA + 2.0 * configs.C44_GPA / a) * 1e9, e6=e6))
-    return reqs
+    """The serving stream's requests (configs.serving_requests)."""
+    return configs.serving_requests(n_requests, seed)
 
 
 def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
@@ -200,8 +178,8 @@ def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
             now = time.perf_counter() - t0
             while k < n_requests and arrivals[k] <= now:
                 q = reqs[k]
-                sv.submit(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], configs.C12_GPA * 1e9,
-                          configs.C44_GPA * 1e9, q["e6"], precision=L.PREC_MIXED, replace_every=50)
+                sv.submit(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], q["c12"],
+                          q["c44"], q["e6"], precision=L.PREC_MIXED, replace_every=50)
                 k += 1
             sv.progress()
             done += sv.take()

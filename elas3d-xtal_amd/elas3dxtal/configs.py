@@ -66,3 +66,29 @@ def six_unit_strains(magnitude=AML_DEFAULT):
     for k in range(3, 6):
         e[k, k] = 0.5 * magnitude
     return e
+
+
+def serving_requests(n_requests, seed=5000):
+    """The serving stream's requests, each with its own seeded microstructure: grid edge
+    uniform over {64, 96, 128, 160, 200}, 10-800 Voronoi grains with uniform SO(3)
+    orientations, Zener ratio A uniform on [0.5, 5], porosity uniform on [0, 5] % as
+    lognormal spheres (median radius 3 voxels, sigma 0.4), one random unit macro strain
+    of magnitude 1e-3 (tensor shear 0.5e-3).  `seed` draws the parameters and request k's
+    microstructure is generated with seed + k.  Returns dicts
+    (shape, nphase, pix, rod, c11, c12, c44 in Pa, e6)."""
+    rng = np.random.default_rng(seed)
+    edges = [64, 96, 128, 160, 200]
+    reqs = []
+    for k in range(n_requests):
+        e = int(edges[int(rng.integers(0, len(edges)))])
+        ng = int(rng.integers(10, 801))
+        frac = float(rng.uniform(0.0, 0.05))
+        pix, rod, _, _ = io.generate((e, e, e), ng, seed + k, pore_mode=io.PORES_SPHERES,
+                                     pore_fraction=frac, pore_median=3.0, pore_sigma=0.4)
+        a = float(rng.uniform(0.5, 5.0))
+        e6 = np.zeros(6)
+        j = int(rng.integers(0, 6))
+        e6[j] = 1e-3 if j < 3 else 0.5e-3
+        reqs.append(dict(shape=(e, e, e), nphase=ng + 1, pix=pix, rod=rod, zener=a,
+                         c11=c11_for_zener(a) * 1e9, c12=C12_GPA * 1e9, c44=C44_GPA * 1e9, e6=e6))
+    return reqs

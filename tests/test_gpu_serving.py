@@ -86,6 +86,36 @@ def test_cobatched_requests_equal_solves_alone():
         _check_same(got[i], _alone(q))
 
 
+def test_serving_stream_of_production_grids():
+    """The serving workload's request mix at production grid sizes (edges 64-200, 10-800
+    grains, Zener ratio 0.5-5, 0-5 % spherical pores, one unit strain each, mixed
+    precision with replacement every 50): the serving benchmark's 256 requests, queued at
+    once and served with co-batching of the small grids, all converge; every fourth equals
+    its solve alone bit for bit (iterations, energy, average stress and strain)."""
+    from elas3dxtal import configs
+    qs = configs.serving_requests(256)
+    edges = sorted({q["shape"][0] for q in qs})
+    print("edges", edges, "grains", min(q["nphase"] for q in qs) - 1, "-", max(q["nphase"] for q in qs) - 1,
+          flush=True)
+    assert len(edges) >= 4
+    kw = dict(precision=L.PREC_MIXED, replace_every=50)
+    with Server(max_active=3, batch_members=6) as sv:
+        ids = [sv.submit(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], q["c12"], q["c44"], q["e6"], **kw)
+               for q in qs]
+        sv.drain()
+        got = {r["id"]: r for r in sv.take()}
+    assert sorted(got) == ids
+    for i in ids:
+        assert got[i]["status"] == L.E3X_OK and got[i]["exit"] == L.PCG_CONVERGED, i
+    for i, q in list(zip(ids, qs))[::4]:
+        r = solve(q["pix"], q["rod"], q["shape"], q["nphase"], q["c11"], q["c12"], q["c44"], q["e6"], **kw)
+        g = got[i]
+        assert g["status"] == L.E3X_OK and g["exit"] == int(r.exit[0]) == L.PCG_CONVERGED, (i, q["shape"])
+        assert g["lstep"] == int(r.lstep[0]) and g["utot"] == float(r.utot[0]), (i, q["shape"])
+        assert np.array_equal(g["stress6"], r.stress6[0]) and np.array_equal(g["strain6"], r.strain6[0])
+        assert np.all(np.isfinite(g["stress6"])) and g["utot"] > 0
+
+
 def test_batching_off_gives_the_same_results():
     qs = [dict(q, shape=(12, 12, 12)) for q in _requests(4, seed=16)]
     for k, q in enumerate(qs):
