@@ -93,12 +93,32 @@ def case_cube200(zener):
     ns = int(np.prod(shape))
     idx = np.arange(0, ns, CUBE200_STRIDE)
     u = r["u"].reshape(3, ns)[:, idx]
+    # The oracle's own sensitivity to rounding: the same PCG from its initial gradient
+    # perturbed by one ulp (alternating signs), run for the same number of iterations.  On
+    # ill-conditioned cases (A = 5) the iterates then move far above 1e-10, and any other
+    # summation order (the GPU's) moves them as much.
+    cm = oc.cmod(w["nphase"], oc.rotate_stiffness(rod, c11, c12, c44), [0.0, 0.0])
+    dk = oc.dk(cm)
+    b, cval = oc.femat_rhs(shape, pix, dk, e6)
+    u0 = oc.affine(shape, e6)
+    g0, _ = oc.energy(shape, pix, dk, b, cval, u0)
+    sign = np.where(np.arange(g0.size) % 2 == 0, 1.0, -1.0)
+    rp = oc.dembx(shape, pix, dk, u0, g0 * (1.0 + 2.0 ** -53 * sign), oc.sumsq(g0), tol=0.0, ldemb=r["lstep"])
+    _, ut_p = oc.energy(shape, pix, dk, b, cval, rp["u"])
+    st_p, _ = oc.stress(shape, pix, cm, e6, rp["u"])
+
+    def rel(a, b_):
+        return float(np.max(np.abs(np.asarray(a) - np.asarray(b_))) / np.max(np.abs(np.asarray(b_))))
+
+    spread = np.array([rel(rp["u"].reshape(3, ns)[:, idx], u), rel(rp["history"], r["history"]),
+                       abs(ut_p - r["utot"]) / abs(r["utot"]), rel(st_p, r["stress6"])])
     return dict(shape=np.array(shape), nphase=w["nphase"], zener=zener, c=np.array([c11, c12, c44]),
                 pix_sha256=hashlib.sha256(np.ascontiguousarray(pix, np.int32).tobytes()).hexdigest(),
                 rod_sha256=hashlib.sha256(np.ascontiguousarray(rod, np.float64).tobytes()).hexdigest(),
                 strain6=e6, lstep=r["lstep"], status=r["status"], reset_count=r["reset_count"],
                 utot0=r["utot0"], gg0=r["gg0"], utot=r["utot"], gg=r["gg"], stress6=r["stress6"],
-                strain_avg6=r["strain6"], history=r["history"], u_index=idx, u_sample=u)
+                strain_avg6=r["strain6"], history=r["history"], u_index=idx, u_sample=u,
+                spread_u_history_utot_stress=spread)
 
 
 CASES = {"poly_10x9x8": case_poly, "reset_6x6x6": case_reset, "void_12x12x10": case_void}
