@@ -114,6 +114,27 @@ using ctab = const __attribute__((address_space(4))) T;
 template <typename T>
 __device__ __forceinline__ ctab<T> *as_table(const T *p) { return (ctab<T> *)p; }
 
+// Scalar-cache prefetch of a coefficient slice of BYTES bytes: one dword every 64 bytes
+// (the line size; the last dword read lies inside the slice),
+// all issued before a single wait, so that the slice's cache misses overlap in one round
+// trip.  Without it the misses come one group at a time: the compiler splits a slice into
+// several scalar loads, each waited for with lgkmcnt(0), which also drains the LDS reads.
+// The empty asm consumes the folded value, which keeps the loads.  Used by the FP64 body,
+// whose slices are twice the bytes of the FP32 ones (the FP32 body gains nothing from it).
+template <int BYTES>
+__device__ __forceinline__ void touch_lines(const void *p)
+{
+    constexpr int N = (BYTES - 4) / 64 + 1;
+    const __attribute__((address_space(4))) unsigned *w = (const __attribute__((address_space(4))) unsigned *)p;
+    unsigned v[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) v[i] = w[i * 16];
+    unsigned x = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) x ^= v[i];
+    asm volatile("" ::"s"(x));
+}
+
 // Component c of neighbour nb = (dz+1)*9 + (dy+1)*3 + (dx+1) relative to the LDS centre hr.
 #define E3X_NB(hr, c, nb) (hr)[(c) * HN + ((nb) / 9 - 1) * (HX * HY) + ((nb) / 3 % 3 - 1) * HX + ((nb) % 3 - 1)]
 
@@ -296,6 +317,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
             if (pm == 0ull) break;
             const int P = __builtin_amdgcn_readlane(pv[0], __builtin_ctzll(pm));
             ctab<T> *S = as_table(sten) + (int64_t)P * kStenHalf;
+            if constexpr (sizeof(T) == 8) touch_lines<kStenHalf * 8>(sten + (int64_t)P * kStenHalf);
             T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
             for (int k = 0; k < 13; k++) {
@@ -349,6 +371,12 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
                     for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
             }
             const int pa = pc[-(PZ(oct) * (VX * VY) + PY(oct) * VX + PX(oct))];
+            if constexpr (sizeof(T) == 8) {
+                // the slice of the octant's first phase (those of the others mostly follow
+                // from the octant before, or share its lines)
+                const int P0 = __builtin_amdgcn_readlane(pa, __builtin_ctzll(__ballot(mlane)));
+                touch_lines<72 * 8>(dk + (int64_t)P0 * 576 + a * 72);
+            }
             bool pend = mlane;
             while (true) {
                 const unsigned long long pm = __ballot(pend);
