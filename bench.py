@@ -187,6 +187,8 @@ def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
                 time.sleep(1e-4)
         wall = time.perf_counter() - t0
     lat = np.array([r["t_done"] - r["t_submit"] for r in done])
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in done])
+    service = np.array([r["t_done"] - r["t_admit"] for r in done])
     ok = sum(r["status"] == 0 and r["exit"] == L.PCG_CONVERGED for r in done)
     dofups = sum(3.0 * np.prod(r["shape"]) * r["lstep"] for r in done)
     by_edge = {}
@@ -202,6 +204,8 @@ def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
         "solves_per_s": n_requests / wall,
         "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
         "latency_ms_p99": float(np.percentile(lat, 99) * 1e3),
+        "queue_wait_ms_p50_p99": [round(float(np.percentile(wait, q) * 1e3), 1) for q in (50, 99)],
+        "admitted_to_done_ms_p50_p99": [round(float(np.percentile(service, q) * 1e3), 1) for q in (50, 99)],
         "dof_updates_per_s": float(dofups / wall),
         "wall_s": wall,
         "microstructures": "one per request, seeded, generated before the stream",
