@@ -156,6 +156,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
 {
     constexpr bool FUSED = MODE == kApplyFused;
     constexpr bool HAS_ST = MODE == kApply || MODE == kApplyFused || MODE == kResid;
+    constexpr bool DIRECT = sizeof(T) == 8 && (MODE == kApply || MODE == kApplyFused);
     __shared__ T hs[3 * HN];
     __shared__ T as[3 * kSB];  // Ah in x-fastest tile order
     __shared__ int ps[VN];
@@ -398,6 +399,38 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
     }
 
     // --- store ---------------------------------------------------------------------
+    if constexpr (DIRECT) {
+        // FP64 PCG apply: each lane stores its own node's Ah and leaves only the node's
+        // h.Ah in LDS, at the node's x-fastest place; the last wave of the block sums those
+        // as the tile epilogue below does (each lane over its four nodes in turn, then the
+        // wave), so the partial is the same bits.  The block holds 18 KB of LDS instead of
+        // 22, so more blocks stay resident while a block's slowest (mixed) wave finishes.
+        __shared__ double prs[kSB];
+        if (active) {
+            const int64_t m = o + 3 * (int64_t)((ox + lx) + nx * (oy + ly) + nxy * (oz + lz));
+            y[m] = acc0;
+            y[m + 1] = acc1;
+            y[m + 2] = acc2;
+        }
+        double pr = 0.0;
+        pr += (double)hr[0] * (double)acc0;
+        pr += (double)hr[HN] * (double)acc1;
+        pr += (double)hr[2 * HN] * (double)acc2;
+        prs[lx + TX * (ly + TY * lz)] = pr;
+        if (!last_wave<kSB>(&arrived)) return;
+        const int l = tid & 63;
+        double p0 = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSB / 64; k++) {
+            const int n = k * 64 + l;
+            const int ex = n % TX, ey = (n / TX) % TY, ez = n / (TX * TY);
+            if (ox + ex >= nx || oy + ey >= ny || oz + ez >= nz) continue;
+            p0 += prs[n];
+        }
+        const double t0 = wave_sum(p0);
+        if (l == 63) A.partials[(int64_t)r * ntile + bid] = t0;
+        return;
+    }
     // Ah into LDS at each lane's node in x-fastest tile order; the last wave of the block
     // to get here stores the tile row by row (the lanes hold their nodes in phase order)
     // and forms the partial sums, each lane over its four nodes in turn, then the wave.
