@@ -281,6 +281,8 @@ def main():
     ap.add_argument("--serve-interarrival-ms", type=float, default=70.0)
     ap.add_argument("--serve-max-active", type=int, default=3)
     ap.add_argument("--serve-batch-members", type=int, default=6, help="co-batching of small grids (1: off)")
+    ap.add_argument("--serve-sweep-ms", type=float, nargs="*", default=[100.0, 85.0, 0.0],
+                    help="further mean interarrival gaps for the latency-against-load sweep (0: all at once)")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run of the apply kernel and a calibration copy, for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -384,6 +386,16 @@ def main():
         reqs = serving_requests(args.serve_requests)
         out["serving"] = run_serving(reqs, args.serve_interarrival_ms, args.serve_max_active,
                                      batch_members=args.serve_batch_members)
+        # the same requests and arrival draws at other offered loads, and all at once
+        # (saturation throughput): latency against load
+        sweep = []
+        for ia in args.serve_sweep_ms:
+            r = run_serving(reqs, ia, args.serve_max_active, batch_members=args.serve_batch_members)
+            sweep.append({"mean_interarrival_ms": ia, "offered_per_s": (1e3 / ia) if ia > 0 else None,
+                          "solves_per_s": round(r["solves_per_s"], 2), "converged": r["converged"],
+                          "latency_ms_p50": round(r["latency_ms_p50"], 1), "latency_ms_p99": round(r["latency_ms_p99"], 1),
+                          "queue_wait_ms_p50_p99": r["queue_wait_ms_p50_p99"]})
+        out["serving"]["load_sweep"] = sweep
         del reqs
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, c11, c12, c44, e6, args.cpu_iters)
