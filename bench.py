@@ -39,9 +39,11 @@ HBM_PEAK_GBS = 8000.0
 # second iteration on), averaged over the two forms the PCG alternates between: every
 # launch reads z, h_prev and the uint16 phase id and writes h and Ah; every other launch
 # also takes two u steps (reads u and h_{it-2}, writes u).  z, h, Ah are FP64 (FP32 in the
-# mixed form); u is FP64 in both.  FP64: (98 + 170) / 2; mixed: (50 + 110) / 2.
+# mixed form); the u steps go to u itself (FP64) or, in the mixed form, to an FP32
+# increment folded into u at each true-gradient replacement.  FP64: (98 + 170) / 2;
+# mixed: (50 + 86) / 2.
 def _apply_bytes(s):
-    return 4 * 3 * s + 2 + (2 * 3 * 8 + 3 * s) / 2
+    return 4 * 3 * s + 2 + (2 * 3 * s + 3 * s) / 2
 
 
 APPLY_BYTES_PER_NODE = {L.PREC_FP64: _apply_bytes(8), L.PREC_MIXED: _apply_bytes(4)}

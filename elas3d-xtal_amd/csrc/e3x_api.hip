@@ -47,7 +47,7 @@ void release(Solver &s)
 {
     (void)hipSetDevice(s.device);
     void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.perm, s.u, s.g, s.b, s.h, s.h_alt, s.z,
-                   s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.cconst, s.partials,
+                   s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.du32, s.cconst, s.partials,
                    s.st, s.hist, s.results, s.post, s.hot, s.hphase, s.minv_hom, s.minv_hom32};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -88,12 +88,14 @@ int ensure_mixed_buffers(Solver &s)
     E3X_HIP(dalloc(s, &s.z32, nv));
     E3X_HIP(dalloc(s, &s.ah32, nv));
     E3X_HIP(dalloc(s, &s.minv32, 6 * s.ns * s.member_count()));
+    E3X_HIP(dalloc(s, &s.du32, nv));
     // the buffers a fused apply reads before any PCG has written them (z and the
     // direction of iteration 1, e.g. under e3x_time_apply) start as zeros, not as stale
     // bits that lambda = 0 would turn into NaN
     E3X_HIP(hipMemsetAsync(s.z32, 0, sizeof(float) * (size_t)nv, s.stream));
     E3X_HIP(hipMemsetAsync(s.h32, 0, sizeof(float) * (size_t)nv, s.stream));
     E3X_HIP(hipMemsetAsync(s.h32_alt, 0, sizeof(float) * (size_t)nv, s.stream));
+    E3X_HIP(hipMemsetAsync(s.du32, 0, sizeof(float) * (size_t)nv, s.stream));
     return E3X_OK;
 }
 
@@ -636,6 +638,8 @@ int e3x_dembx_begin(e3x_solver *h, int precision, int replace_every, const doubl
     if (mixed) {
         E3X_HIP(launch_to_f32(s.g, s.g32, nv, s.stream));
         if (kkk != 1) E3X_HIP(launch_to_f32(s.h, s.h32, nv, s.stream));
+        // the u steps start from zero (a run that ended in an error may have left some)
+        E3X_HIP(hipMemsetAsync(s.du32, 0, sizeof(float) * (size_t)nv, s.stream));
     }
     E3X_HIP(launch_pcg_init(s, mixed, kkk, ldemb, tol, gg0));
     Solver::PcgRun &R = s.run;

@@ -31,6 +31,11 @@ constexpr int VX = TX + 1, VY = TY + 1, VZ = TZ + 1, VN = VX * VY * VZ;  // voxe
 // (9 each) then the centre block's upper triangle 00 11 22 01 02 12.
 constexpr int kStenHalf = 13 * 9 + 6;
 
+// What the u steps of a PCG in vectors of type T accumulate into: u itself (FP64), or in
+// the mixed form the FP32 increment du32.
+template <typename T>
+using ustep_t = std::conditional_t<sizeof(T) == 4, float, double>;
+
 // Per right-hand-side PCG scalars, resident on the device (the locals of
 // dembx_OpenMP, f90:1440-1445).
 struct PcgState {
@@ -117,6 +122,9 @@ struct Solver {
     double *minv_hom = nullptr;
     float *minv_hom32 = nullptr;
     float *g32 = nullptr, *h32 = nullptr, *z32 = nullptr, *ah32 = nullptr, *minv32 = nullptr;
+    // mixed form: the u steps since the last true-gradient replacement, in FP32; folded
+    // into u (FP64) before each replacement and after the loop (launch_fold_du)
+    float *du32 = nullptr;
     // Second direction buffer: the direction of PCG iteration k lives in h (k odd) or
     // h_alt (k even), since the apply that forms it reads the previous one as halo.
     // h holds the final direction after e3x_dembx.
@@ -206,6 +214,9 @@ hipError_t launch_pcg_init(Solver &s, bool mixed, int kkk, int ldemb, float tol,
 hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, int replace_every);
 // after the loop: the step still owed to u and the final direction, into h (h32)
 hipError_t launch_pcg_finalize(Solver &s, bool mixed, int replace_every);
+// mixed form: u += du32, du32 = 0 for the right-hand sides still running (running_only)
+// or for all
+hipError_t launch_fold_du(Solver &s, bool running_only);
 // e3x_post.hip
 hipError_t launch_stress(Solver &s);                          // results[0..12*nrhs)
 hipError_t launch_fullfield(Solver &s, int rhs, double *stress_dev, double *vm_dev);

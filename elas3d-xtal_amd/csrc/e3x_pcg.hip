@@ -37,7 +37,7 @@ struct PcgArgs {
     int64_t ns;
     int it, ldemb, kkk;
     int usteps;     // DO_U: 1, or 2 with the step of iteration it-1 (from hm1) first
-    double *u;
+    ustep_t<T> *u;  // u (FP64 form) or the FP32 u steps since the last replacement (mixed)
     T *g, *h, *z, *ah;
     const T *hm1;   // h_{it-1}
     const T *minv;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_hah_stage(const double *partials, in
 // also the partials of g.g, in slots [NR + r] (E3X_STOP_NIST_ABS).  MEM: the right-hand
 // sides are the members of a batch, each with its own M^-1 (read per right-hand side).
 template <typename T, int NR, bool UPDATE_G, bool DO_U, bool DO_Z, bool GSQ = false, bool MEM = false>
-__device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__restrict__ u, T *__restrict__ gv,
+__device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, ustep_t<T> *__restrict__ u, T *__restrict__ gv,
                                                 const T *__restrict__ hv, T *__restrict__ zv,
                                                 const T *__restrict__ ah, const T *__restrict__ minv)
 {
@@ -201,9 +201,9 @@ __device__ __forceinline__ void pcg_update_body(const PcgArgs<T> &A, double *__r
             if constexpr (DO_U) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    double uc = u[o + c];
+                    double uc = (double)u[o + c];
                     if (A.usteps == 2) uc = uc - lamp[r] * (double)A.hm1[o + c];
-                    u[o + c] = uc - lam[r] * (double)hv[o + c];
+                    u[o + c] = (ustep_t<T>)(uc - lam[r] * (double)hv[o + c]);
                 }
             }
             double g[3];
@@ -329,10 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_finalize(PcgArgs<T> A, T *h_alt,
                 const int64_t o = ((int64_t)r * ns + m) * 3 + c;
                 const T hv = hk[o];
                 if (first[r] <= last[r]) {
-                    double uc = A.u[o];
+                    double uc = (double)A.u[o];
                     if (first[r] <= K[r] - 1) uc = uc - lamK1[r] * (double)hk1[o];  // step K-1
                     if (last[r] == K[r]) uc = uc - lamK[r] * (double)hv;            // step K
-                    A.u[o] = uc;
+                    A.u[o] = (ustep_t<T>)uc;
                 }
                 if (dir[r] == 1) A.h[o] = rst[r] ? A.z[o] : (T)((double)A.z[o] + gam[r] * (double)hv);
                 else if (dir[r] == 2 && hk != A.h) A.h[o] = hv;
@@ -345,7 +345,9 @@ template <typename T>
 PcgArgs<T> pcg_args(const Solver &s, int it)
 {
     PcgArgs<T> A{};
-    A.ns = s.ns; A.it = it; A.u = s.u;
+    A.ns = s.ns; A.it = it;
+    if constexpr (sizeof(T) == 4) A.u = s.du32;
+    else A.u = s.u;
     A.partials = s.partials; A.st = s.st; A.hist = s.hist;
     A.hphase = s.hphase;
     A.wide = s.wide ? 1 : 0;
@@ -437,6 +439,7 @@ void iter_nr(Solver &s, int it, int replace_every)
         }
         A.usteps = it - u_done_after_apply(it, replace_every);
         launch_update<T, NR, false, true, false>(s, vg, A);
+        if constexpr (sizeof(T) == 4) (void)launch_fold_du(s, true);  // u now has every step
         (void)launch_resid(s);
         launch_update<T, NR, false, false, true>(s, vg, A);
     }
@@ -540,10 +543,36 @@ hipError_t launch_pcg_iter(Solver &s, bool mixed, int it, int replace_every)
     return hipGetLastError();
 }
 
+// Mixed form: u += du, du = 0 over the right-hand sides' rows (those still running, or all).
+__global__ __launch_bounds__(kBlock) void k_fold_du(double *u, float *du, const PcgState *st, int64_t n3,
+                                                  int running_only)
+{
+    const int r = blockIdx.y;
+    if (running_only && st[r].stop_iter != 0) return;  // uniform over the block
+    double *ur = u + (int64_t)r * n3;
+    float *dr = du + (int64_t)r * n3;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
+        ur[i] = ur[i] + (double)dr[i];
+        dr[i] = 0.0f;
+    }
+}
+
+hipError_t launch_fold_du(Solver &s, bool running_only)
+{
+    const int64_t n3 = 3 * s.ns;
+    const int64_t g = std::min<int64_t>((n3 + kBlock - 1) / kBlock, 2048);
+    hipLaunchKernelGGL(k_fold_du, dim3((unsigned)g, (unsigned)s.nrhs), dim3(kBlock), 0, s.stream, s.u, s.du32,
+                       (const PcgState *)s.st, n3, running_only ? 1 : 0);
+    return hipGetLastError();
+}
+
 hipError_t launch_pcg_finalize(Solver &s, bool mixed, int replace_every)
 {
-    if (mixed) finalize_t<float>(s, replace_every);
-    else finalize_t<double>(s, 0);
+    if (mixed) {
+        finalize_t<float>(s, replace_every);
+        return launch_fold_du(s, false);  // u complete for every right-hand side
+    }
+    finalize_t<double>(s, 0);
     return hipGetLastError();
 }
 

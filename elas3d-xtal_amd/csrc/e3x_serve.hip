@@ -77,11 +77,11 @@ struct Server {
 Server &SV(e3x_server *p) { return *reinterpret_cast<Server *>(p); }
 const Server &SV(const e3x_server *p) { return *reinterpret_cast<const Server *>(p); }
 
-// The mixed form's FP32 buffers (ensure_mixed_buffers): g h h_alt z Ah and M^-1.
+// The mixed form's FP32 buffers (ensure_mixed_buffers): g h h_alt z Ah du and M^-1.
 int64_t mixed_bytes(int nx, int ny, int nz)
 {
     const int64_t ns = (int64_t)nx * ny * nz;
-    return 5 * 3 * ns * 4 + 6 * ns * 4;
+    return 6 * 3 * ns * 4 + 6 * ns * 4;
 }
 
 // Device bytes of a one-rhs handle created for nphase_cap phases, with the mixed form's

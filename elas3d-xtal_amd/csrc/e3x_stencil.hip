@@ -67,7 +67,7 @@ struct StencilArgs {
     // then u -= lambda hold (gamma, reset, lambdas of the previous iterations)
     const T *z, *hold;
     T *hnew;
-    double *u;
+    ustep_t<T> *u;         // u (FP64), or the mixed form's FP32 u steps since the last replacement
     int usteps;
     int check_first;       // read the stop state before loading (launches past the predicted stop)
     int rmap[kMaxRhs];     // grid row j serves right-hand side rmap[j]
@@ -152,7 +152,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
                                              const T *__restrict__ x, T *__restrict__ y,
                                              const double *__restrict__ bvec, float *__restrict__ y32,
                                              const T *__restrict__ zv, const T *__restrict__ hold,
-                                             T *__restrict__ hnew, double *__restrict__ u)
+                                             T *__restrict__ hnew, ustep_t<T> *__restrict__ u)
 {
     constexpr bool FUSED = MODE == kApplyFused;
     constexpr bool HAS_ST = MODE == kApply || MODE == kApplyFused || MODE == kResid;
@@ -249,7 +249,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
         if (ingrid && A.usteps != 0) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                ui[c] = u[gi + c];
+                ui[c] = (double)u[gi + c];
                 hm2[c] = hnew[gi + c];  // before this apply overwrites it with h_it
             }
         }
@@ -280,7 +280,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
             if (ingrid) {
                 if (dou) {
                     const double u1 = ui[c] - lamp * (double)hm2[c];
-                    u[gi + c] = u1 - lam * (double)bi[c];
+                    u[gi + c] = (ustep_t<T>)(u1 - lam * (double)bi[c]);
                 }
                 hnew[gi + c] = hv;
             }
@@ -742,7 +742,9 @@ void apply_iter(Solver &s, const T *dk, const T *sten, T *hbuf[2], const T *z, T
         dispatch_stencil<T, kApply>(s, A, rows);
     } else {
         A.z = z; A.hold = hbuf[(it - 1) & 1]; A.hnew = hbuf[it & 1];
-        A.u = s.u; A.usteps = usteps;
+        if constexpr (sizeof(T) == 4) A.u = s.du32;
+        else A.u = s.u;
+        A.usteps = usteps;
         A.check_first = s.pcg_check_first ? 1 : 0;
         dispatch_stencil<T, kApplyFused>(s, A, rows);
     }

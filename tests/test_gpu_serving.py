@@ -143,7 +143,7 @@ def _handle_bytes(shape, mixed, ldemb=10000, phases=1024):
     tiles = -(-shape[0] // 8) * -(-shape[1] // 8) * -(-shape[2] // 4)
     b += max(12 * 2048, 2 * tiles) * 8 + (1 << 16) + ldemb * 8
     if mixed:
-        b += 5 * 3 * ns * 4 + 6 * ns * 4
+        b += 6 * 3 * ns * 4 + 6 * ns * 4     # g h h_alt z Ah du, M^-1 (FP32)
     return b
 
 
