@@ -84,8 +84,10 @@ enum e3x_field {
 /* Arithmetic of e3x_dembx_ex. */
 enum e3x_precision {
     E3X_PREC_FP64 = 0,  /* the reference's arithmetic                                   */
-    E3X_PREC_MIXED = 1  /* FP32 stencil, vectors and block-Jacobi; FP64 u, dots, norms
-                           and a true-residual replacement every `replace_every` steps */
+    E3X_PREC_MIXED = 1  /* FP32 stencil, vectors and block-Jacobi, the u steps in an FP32
+                           increment folded into the FP64 u at each true-residual
+                           replacement (every `replace_every` steps) and at the end; FP64
+                           dots, norms and replacement                                  */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
