@@ -215,6 +215,30 @@ int solver_set_nphase(Solver &s, int nphase)
     return E3X_OK;
 }
 
+// e3x_set_microstructure for phase ids already checked and converted (0-based, in the
+// handle's width) with their histogram: no pass over the labels and no wait, for the
+// request loop, whose preparation thread does that work.  ids and orientation must stay
+// valid until the stream has run the copies.
+int solver_set_microstructure_ids(Solver &s, const void *ids, bool wide_ids, const int64_t *counts,
+                                  const double *orientation)
+{
+    if (s.run.on) return fail(E3X_ESTATE, "a PCG is in progress on this handle");
+    if (s.members) return fail(E3X_ESTATE, "not for a batch handle (e3x_set_member)");
+    if (!ids || !counts) return fail(E3X_EINVAL, "ids or counts is NULL");
+    if (wide_ids != s.wide) return fail(E3X_EINVAL, "phase-id width does not match the handle");
+    s.counts.assign(counts, counts + s.nphase);
+    E3X_HIP(hipSetDevice(s.device));
+    E3X_HIP(hipMemcpyAsync(s.pix, ids, (s.wide ? sizeof(uint32_t) : sizeof(uint16_t)) * (size_t)s.ns,
+                           hipMemcpyHostToDevice, s.stream));
+    if (orientation && s.nphase > 1)
+        E3X_HIP(hipMemcpyAsync(s.orient, orientation, sizeof(double) * 3 * (size_t)(s.nphase - 1),
+                               hipMemcpyHostToDevice, s.stream));
+    E3X_HIP(launch_tile_order(s));
+    s.have_pix = true;
+    s.have_femat = s.have_precond = s.have_precond32 = false;
+    return E3X_OK;
+}
+
 int report_error(int code, const char *msg)
 {
     return fail(code, msg ? msg : "");

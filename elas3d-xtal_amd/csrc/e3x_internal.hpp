@@ -246,6 +246,10 @@ hipError_t launch_hotspots(Solver &s, int rhs, int nsolid, double sigma, float t
 // phases (the serving loop's handle pool); the microstructure and tables must be
 // loaded again.  Returns an e3x_status.
 int solver_set_nphase(Solver &s, int nphase);
+// e3x_set_microstructure from prepared 0-based phase ids (uint16, or uint32 on a wide
+// handle) and their histogram; asynchronous (e3x_serve.hip)
+int solver_set_microstructure_ids(Solver &s, const void *ids, bool wide_ids, const int64_t *counts,
+                                  const double *orientation);
 // e3x_api.hip: a handle for nrhs right-hand sides sharing one microstructure (members = 0)
 // or for `members` separate problems (nrhs = members); an e3x_status
 int solver_create(int nx, int ny, int nz, int nphase, int nrhs, int members, int device, Solver **out);

@@ -15,11 +15,15 @@
 #include "e3x_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <list>
 #include <iterator>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -32,13 +36,98 @@ namespace {
 // Phases a serving handle is created for: any request with at most this many (grains
 // plus the pore phase) can reuse it.
 constexpr int kPoolPhases = 1024;
+// Requests of at most this many nodes are prepared by e3x_server_submit itself; larger
+// ones by the preparation thread, and progress admits them once they are ready.
+constexpr int64_t kPrepInline = 64 * 64 * 64;
+
+// A request's microstructure and what the server's preparation thread derives from it
+// before admission: the label check (f90:2986-2989), the phase histogram and the 0-based
+// phase ids in the width of the handle that will take it.  Shared with that thread.
+struct Prep {
+    std::vector<int32_t> pix;
+    int nphase = 0;
+    std::atomic<int> state{0};  // 0 pending, 1 ready, -1 a label out of [1, nphase]
+    std::vector<uint16_t> id16;
+    std::vector<uint32_t> id32;
+    std::vector<int64_t> counts;
+};
 
 struct Request {
     int64_t id = 0;
     e3x_request rq{};
-    std::vector<int32_t> pix;
+    std::shared_ptr<Prep> prep;
     std::vector<double> ori;
     double t_submit = 0;
+};
+
+// The preparation of one request: a single pass over its labels.
+void prepare(Prep &p)
+{
+    const int64_t ns = (int64_t)p.pix.size();
+    const bool wide = p.nphase > kNarrowPhases;
+    p.counts.assign((size_t)p.nphase, 0);
+    if (wide) p.id32.resize((size_t)ns);
+    else p.id16.resize((size_t)ns);
+    bool ok = true;
+    for (int64_t m = 0; m < ns; m++) {
+        const int32_t v = p.pix[(size_t)m];
+        if (v < 1 || v > p.nphase) {
+            ok = false;
+            break;
+        }
+        p.counts[(size_t)(v - 1)]++;
+        if (wide) p.id32[(size_t)m] = (uint32_t)(v - 1);
+        else p.id16[(size_t)m] = (uint16_t)(v - 1);
+    }
+    if (!ok) {
+        p.id16.clear();
+        p.id32.clear();
+    }
+    p.state.store(ok ? 1 : -1, std::memory_order_release);
+}
+
+// The server's preparation thread: takes submitted requests in order, off the thread
+// that drives the GPU (a 200^3 request is ~10 ms of host work).
+struct Preparer {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<Prep>> todo;
+    bool stop = false;
+    std::thread th;
+
+    Preparer()
+    {
+        th = std::thread([this] {
+            for (;;) {
+                std::shared_ptr<Prep> p;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [this] { return stop || !todo.empty(); });
+                    if (todo.empty()) return;  // stop, and nothing left
+                    p = std::move(todo.front());
+                    todo.pop_front();
+                }
+                prepare(*p);
+            }
+        });
+    }
+    ~Preparer()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_one();
+        th.join();
+    }
+    void push(std::shared_ptr<Prep> p)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            todo.push_back(std::move(p));
+        }
+        cv.notify_one();
+    }
 };
 
 struct Handle {
@@ -67,6 +156,7 @@ struct Server {
     std::vector<Active> active;
     std::list<Handle> pool;  // idle handles, least recently used first
     std::deque<e3x_response> done;
+    Preparer prep;           // label check and phase ids of submitted requests
 
     double now() const
     {
@@ -204,14 +294,10 @@ e3x_response make_response(const Request &r, int status)
     return o;
 }
 
-// Labels of the request inside [1, nphase] (f90:2986-2989), checked on the host before the
-// request joins a solve, so that one bad request does not fail a batch.
-bool labels_ok(const Request &r)
-{
-    for (int32_t v : r.pix)
-        if (v < 1 || v > r.rq.nphase) return false;
-    return true;
-}
+// The preparation of a queued request: 0 pending, 1 ready, -1 a label out of [1, nphase]
+// (f90:2986-2989; found before the request joins a solve, so that one bad request does
+// not fail a batch).
+int prep_state(const Request &r) { return r.prep->state.load(std::memory_order_acquire); }
 
 // Requests that can share a batch: one grid, precision and set of PCG controls.
 bool same_batch(const e3x_request &a, const e3x_request &b)
@@ -232,7 +318,17 @@ int start_solve(Active &a)
     if (a.hd.members == 0) {
         if (int rc = solver_set_nphase(sv, q.nphase)) return rc;
         const Request &r = a.reqs[0];
-        if (int rc = e3x_set_microstructure(h, r.pix.data(), r.ori.empty() ? nullptr : r.ori.data())) return rc;
+        const Prep &p = *r.prep;
+        const double *ori = r.ori.empty() ? nullptr : r.ori.data();
+        // the prepared phase ids when their width is the handle's (a pooled handle made for
+        // more than 65,535 phases takes 32-bit ids); else the labels themselves
+        const bool wide_ids = !p.id32.empty();
+        if (wide_ids == sv.wide && (wide_ids || !p.id16.empty())) {
+            const void *ids = wide_ids ? (const void *)p.id32.data() : (const void *)p.id16.data();
+            if (int rc = solver_set_microstructure_ids(sv, ids, wide_ids, p.counts.data(), ori)) return rc;
+        } else if (int rc = e3x_set_microstructure(h, p.pix.data(), ori)) {
+            return rc;
+        }
         double cval = 0;
         if (int rc = e3x_femat(h, q.c11, q.c12, q.c44, q.phasemod, q.flag_m, q.strain6, &cval)) return rc;
     } else {
@@ -242,7 +338,7 @@ int start_solve(Active &a)
         for (int k = 0; k < n; k++) {
             const Request &r = a.reqs[(size_t)k];
             const e3x_request &rk = r.rq;
-            if (int rc = e3x_set_member(h, k, r.pix.data(), r.ori.empty() ? nullptr : r.ori.data(), rk.c11, rk.c12,
+            if (int rc = e3x_set_member(h, k, r.prep->pix.data(), r.ori.empty() ? nullptr : r.ori.data(), rk.c11, rk.c12,
                                         rk.c44, rk.phasemod, rk.flag_m, rk.strain6))
                 return rc;
         }
@@ -318,7 +414,12 @@ int progress(Server &S, bool *changed)
     // admit queued requests: first fit in arrival order; a request of a small grid takes
     // the queued requests it can share a batch with (in arrival order) along
     for (auto it = S.queue.begin(); it != S.queue.end() && (int)S.active.size() < S.max_active;) {
-        if (!labels_ok(*it)) {
+        const int ps = prep_state(*it);
+        if (ps == 0) {  // not prepared yet
+            ++it;
+            continue;
+        }
+        if (ps < 0) {
             e3x_response o = make_response(*it, report_error(E3X_EPHASE, "phase label in pix out of bounds"));
             o.t_admit = o.t_done = S.now();
             S.done.push_back(o);
@@ -330,7 +431,7 @@ int progress(Server &S, bool *changed)
         std::vector<std::list<Request>::iterator> group{it};
         if (S.batch_members > 1 && (int64_t)q.nx * q.ny * q.nz <= S.batch_max_nodes)
             for (auto jt = std::next(it); jt != S.queue.end() && (int)group.size() < S.batch_members; ++jt)
-                if (same_batch(jt->rq, q) && labels_ok(*jt)) group.push_back(jt);
+                if (same_batch(jt->rq, q) && prep_state(*jt) == 1) group.push_back(jt);
         const int members = group.size() > 1 ? (int)group.size() : 0;
         int np_each = 0;
         for (auto &g : group) np_each = std::max(np_each, g->rq.nphase);
@@ -441,11 +542,15 @@ int e3x_server_submit(e3x_server *sv, const e3x_request *rq, int64_t *id)
     r.id = S.next_id++;
     r.rq = *rq;
     const int64_t ns = (int64_t)rq->nx * rq->ny * rq->nz;
-    r.pix.assign(rq->pix, rq->pix + ns);
+    r.prep = std::make_shared<Prep>();
+    r.prep->pix.assign(rq->pix, rq->pix + ns);
+    r.prep->nphase = rq->nphase;
     if (rq->orientation && rq->nphase > 1) r.ori.assign(rq->orientation, rq->orientation + 3 * (rq->nphase - 1));
     r.rq.pix = nullptr;
     r.rq.orientation = nullptr;
     r.t_submit = S.now();
+    if (ns <= kPrepInline) prepare(*r.prep);  // ready at once (under ~0.3 ms of host work)
+    else S.prep.push(r.prep);
     S.queue.push_back(std::move(r));
     if (id) *id = S.next_id - 1;
     return E3X_OK;

@@ -17,6 +17,12 @@
  * only when memory is needed.  Results do not depend on what else runs: every kernel
  * sums in a fixed order.
  *
+ * The host work on a request's labels (the check against nphase, the phase histogram,
+ * the conversion to the device's phase ids) is done at submit for grids of at most 64^3
+ * nodes, and otherwise by a preparation thread of the server; progress admits a request
+ * once it is prepared.  So the thread that drives the GPU does not stop for it (a 200^3
+ * request is ~10 ms of it) while other solves wait for their next iterations.
+ *
  * One host thread drives a server (submit, progress, take, drain are not thread-safe
  * among themselves); different servers are independent.  Every call returns E3X_OK or
  * an e3x_status with e3x_last_error() set.
