@@ -505,11 +505,11 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 
 // Lane order of each stencil tile (Solver::perm), from the phases alone: one block per
 // tile, thread t = node t of the tile's wave-cube numbering.  Key: homogeneous nodes
-// first, by phase; then mixed nodes by (smallest, largest, octant-0 phase); nodes
-// outside the grid last; ties by t.  A bitonic sort of the kSB keys in LDS.  The phases
-// enter the key as their ranks among the tile's voxel-halo phases (a sorted copy in LDS,
-// at most kSortN of them): the same order as the phases themselves, in 9 bits whatever
-// their width.
+// first, by phase; then mixed nodes by (smallest phase, largest phase, the set of octants
+// holding another phase than the smallest, octant-0 phase); nodes outside the grid last;
+// ties by t.  A bitonic sort of the kSB keys in LDS.  The phases enter the key as their
+// ranks among the tile's voxel-halo phases (a sorted copy in LDS, at most kSortN of
+// them): the same order as the phases themselves, in 10 bits whatever their width.
 constexpr int kSortN = 512;
 static_assert(VN <= kSortN && kSortN % kSB == 0, "voxel halo must fit the rank table");
 
@@ -560,19 +560,25 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
     const int tw = t >> 6, tl = t & 63;
     const int lx = wave_x(tw) + (tl & 3), ly = wave_y(tw) + ((tl >> 2) & 3), lz = wave_z(tw) + (tl >> 4);
     const unsigned *pc = ps + (lz + 1) * (VX * VY) + (ly + 1) * VX + (lx + 1);
-    unsigned pmin = 0xFFFFFFFFu, pmax = 0u;
+    unsigned pmin = 0xFFFFFFFFu, pmax = 0u, pv[8];
     const unsigned p0 = pc[0];
 #pragma unroll
     for (int a = 0; a < 8; a++) {
-        const unsigned pa = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
-        pmin = min(pmin, pa);
-        pmax = max(pmax, pa);
+        pv[a] = pc[-(PZ(a) * (VX * VY) + PY(a) * VX + PX(a))];
+        pmin = min(pmin, pv[a]);
+        pmax = max(pmax, pv[a]);
     }
+    // which octants hold a phase other than the smallest: mixed nodes with the same phases
+    // and the same pattern share every octant's phase, so the wave serves them in one pass
+    // per octant
+    unsigned sig = 0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) sig |= (pv[a] != pmin ? 1u : 0u) << a;
     const bool active = ox + lx < nx && oy + ly < ny && oz + lz < nz;
     const unsigned long long cls = !active ? 2ull : (pmin == pmax ? 0ull : 1ull);
-    key[t] = (cls << 62) | ((unsigned long long)phase_rank(pmin) << 44) |
-             ((unsigned long long)phase_rank(pmax) << 28) | ((unsigned long long)phase_rank(p0) << 12) |
-             (unsigned long long)t;
+    key[t] = (cls << 62) | ((unsigned long long)phase_rank(pmin) << 50) |
+             ((unsigned long long)phase_rank(pmax) << 40) | ((unsigned long long)sig << 32) |
+             ((unsigned long long)phase_rank(p0) << 22) | (unsigned long long)t;
     __syncthreads();
     for (int k = 2; k <= kSB; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
