@@ -162,7 +162,7 @@ def serving_requests(n_requests, seed=5000):
     return configs.serving_requests(n_requests, seed)
 
 
-def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
+def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6, small_slots=0, batch_max_nodes=None):
     """Serving stream over `reqs` with Poisson arrivals (mean gap interarrival_ms; 0: all
     at once).  One host thread submits at the arrival times and drives the server between
     them; latency is arrival to result."""
@@ -172,7 +172,8 @@ def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(interarrival_ms * 1e-3, n_requests) if interarrival_ms > 0 else np.zeros(n_requests)
     arrivals = np.cumsum(gaps) - gaps[0]
-    with Server(max_active=max_active, batch_members=batch_members) as sv:
+    with Server(max_active=max_active, batch_members=batch_members, batch_max_nodes=batch_max_nodes,
+                small_slots=small_slots) as sv:
         t0 = time.perf_counter()
         done = []
         k = 0
@@ -199,7 +200,7 @@ def run_serving(reqs, interarrival_ms, max_active, seed=5001, batch_members=6):
         by_edge[str(e)] = dict(n=int(le.size), p50_ms=round(float(np.percentile(le, 50) * 1e3), 1),
                                p99_ms=round(float(np.percentile(le, 99) * 1e3), 1))
     return {
-        "requests": n_requests, "converged": int(ok), "max_active": max_active,
+        "requests": n_requests, "converged": int(ok), "max_active": max_active, "small_slots": small_slots,
         "co_batching_members": batch_members,
         "latency_by_edge": by_edge,
         "mean_interarrival_ms": interarrival_ms,
@@ -281,7 +282,9 @@ def main():
     ap.add_argument("--six-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=256, help="serving stream length (0: skip)")
     ap.add_argument("--serve-interarrival-ms", type=float, default=70.0)
-    ap.add_argument("--serve-max-active", type=int, default=3)
+    ap.add_argument("--serve-max-active", type=int, default=2, help="serving slots of solves above the co-batching size")
+    ap.add_argument("--serve-small-slots", type=int, default=2,
+                    help="serving slots of their own for co-batched small grids (0: they share --serve-max-active)")
     ap.add_argument("--serve-batch-members", type=int, default=6, help="co-batching of small grids (1: off)")
     ap.add_argument("--serve-sweep-ms", type=float, nargs="*", default=[100.0, 85.0, 0.0],
                     help="further mean interarrival gaps for the latency-against-load sweep (0: all at once)")
@@ -387,12 +390,13 @@ def main():
     if args.serve_requests > 0:
         reqs = serving_requests(args.serve_requests)
         out["serving"] = run_serving(reqs, args.serve_interarrival_ms, args.serve_max_active,
-                                     batch_members=args.serve_batch_members)
+                                     batch_members=args.serve_batch_members, small_slots=args.serve_small_slots)
         # the same requests and arrival draws at other offered loads, and all at once
         # (saturation throughput): latency against load
         sweep = []
         for ia in args.serve_sweep_ms:
-            r = run_serving(reqs, ia, args.serve_max_active, batch_members=args.serve_batch_members)
+            r = run_serving(reqs, ia, args.serve_max_active, batch_members=args.serve_batch_members,
+                            small_slots=args.serve_small_slots)
             sweep.append({"mean_interarrival_ms": ia, "offered_per_s": (1e3 / ia) if ia > 0 else None,
                           "solves_per_s": round(r["solves_per_s"], 2), "converged": r["converged"],
                           "latency_ms_p50": round(r["latency_ms_p50"], 1), "latency_ms_p99": round(r["latency_ms_p99"], 1),

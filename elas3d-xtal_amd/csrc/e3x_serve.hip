@@ -141,6 +141,7 @@ struct Handle {
 struct Active {
     std::vector<Request> reqs;
     Handle hd;
+    bool small = false;  // a grid of at most batch_max_nodes nodes
     double t_admit = 0;
     double utot0[kMaxRhs] = {};
 };
@@ -151,6 +152,7 @@ struct Server {
     std::chrono::steady_clock::time_point t0;
     int64_t next_id = 0;
     int batch_members = kMaxRhs;               // co-batching: at most this many per batch (1: off)
+    int small_slots = 0;                       // slots of their own for grids of at most batch_max_nodes
     int64_t batch_max_nodes = 128 * 128 * 128; // ... of grids with at most this many nodes
     std::list<Request> queue;
     std::vector<Active> active;
@@ -412,8 +414,22 @@ int progress(Server &S, bool *changed)
         *changed = true;
     }
     // admit queued requests: first fit in arrival order; a request of a small grid takes
-    // the queued requests it can share a batch with (in arrival order) along
-    for (auto it = S.queue.begin(); it != S.queue.end() && (int)S.active.size() < S.max_active;) {
+    // the queued requests it can share a batch with (in arrival order) along.  With
+    // small_slots > 0 the solves of small grids have that many slots of their own and the
+    // others max_active; else all share max_active.
+    auto small_grid = [&](const e3x_request &q) { return (int64_t)q.nx * q.ny * q.nz <= S.batch_max_nodes; };
+    auto slot_free = [&](const e3x_request &q) {
+        if (S.small_slots <= 0) return (int)S.active.size() < S.max_active;
+        int nsmall = 0;
+        for (const Active &a : S.active) nsmall += a.small ? 1 : 0;
+        return small_grid(q) ? nsmall < S.small_slots : (int)S.active.size() - nsmall < S.max_active;
+    };
+    for (auto it = S.queue.begin(); it != S.queue.end();) {
+        if (S.small_slots <= 0 && (int)S.active.size() >= S.max_active) break;
+        if (!slot_free(it->rq)) {
+            ++it;
+            continue;
+        }
         const int ps = prep_state(*it);
         if (ps == 0) {  // not prepared yet
             ++it;
@@ -457,6 +473,7 @@ int progress(Server &S, bool *changed)
         }
         Active a;
         a.hd = hd;
+        a.small = small_grid(q);
         a.t_admit = S.now();
         for (auto &g : group) a.reqs.push_back(std::move(*g));
         for (size_t k = 1; k < group.size(); k++) S.queue.erase(group[k]);  // never the current one
@@ -604,6 +621,14 @@ int e3x_server_set_batching(e3x_server *sv, int max_members, int64_t max_nodes)
     Server &S = SV(sv);
     S.batch_members = max_members;
     S.batch_max_nodes = max_nodes;
+    return E3X_OK;
+}
+
+int e3x_server_set_small_slots(e3x_server *sv, int slots)
+{
+    if (!sv) return report_error(E3X_EINVAL, "NULL server");
+    if (slots < 0 || slots > 64) return report_error(E3X_EINVAL, "slots must be in [0, 64]");
+    SV(sv).small_slots = slots;
     return E3X_OK;
 }
 

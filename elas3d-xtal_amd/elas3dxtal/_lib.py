@@ -104,6 +104,7 @@ SERVE_SIGNATURES = {
     "e3x_server_take": (C.c_int, [_vp, C.POINTER(Response), _ip]),
     "e3x_server_drain": (C.c_int, [_vp]),
     "e3x_server_set_batching": (C.c_int, [_vp, C.c_int, C.c_int64]),
+    "e3x_server_set_small_slots": (C.c_int, [_vp, C.c_int]),
     "e3x_server_clock": (C.c_int, [_vp, _dp]),
     "e3x_server_memory": (C.c_int, [_vp, C.POINTER(C.c_int64), _ip]),
 }

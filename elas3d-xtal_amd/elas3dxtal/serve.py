@@ -31,7 +31,8 @@ def _response_dict(r):
 
 
 class Server:
-    def __init__(self, device=0, hbm_budget=0, max_active=8, batch_members=None, batch_max_nodes=None):
+    def __init__(self, device=0, hbm_budget=0, max_active=8, batch_members=None, batch_max_nodes=None,
+                 small_slots=0):
         self._lib = L.load()
         h = C.c_void_p()
         L.check(self._lib.e3x_server_create(device, int(hbm_budget), int(max_active), C.byref(h)))
@@ -40,12 +41,19 @@ class Server:
         if batch_members is not None or batch_max_nodes is not None:
             self.set_batching(6 if batch_members is None else batch_members,
                               128 ** 3 if batch_max_nodes is None else batch_max_nodes)
+        if small_slots:
+            self.set_small_slots(small_slots)
 
     def set_batching(self, max_members, max_nodes):
         """Co-batching of small grids (e3x_server_set_batching): up to max_members queued
         requests on one grid of at most max_nodes nodes (same precision and PCG controls)
         share one batch handle; max_members = 1 turns it off."""
         L.check(self._lib.e3x_server_set_batching(self._h, int(max_members), int(max_nodes)))
+
+    def set_small_slots(self, slots):
+        """Slots of their own for the solves of small grids (e3x_server_set_small_slots):
+        up to `slots` beside the max_active ones; 0 shares max_active."""
+        L.check(self._lib.e3x_server_set_small_slots(self._h, int(slots)))
 
     def close(self):
         if self._h is not None and self._h.value:

@@ -99,6 +99,14 @@ int e3x_server_drain(e3x_server *sv);
  */
 int e3x_server_set_batching(e3x_server *sv, int max_members, int64_t max_nodes);
 
+/*
+ * Slots of their own for the solves of small grids (at most the co-batching max_nodes):
+ * with slots > 0 they take up to `slots` slots beside the max_active ones, which the
+ * other solves keep for themselves, so a small request does not queue behind large
+ * solves.  0 (the default): every solve shares max_active.
+ */
+int e3x_server_set_small_slots(e3x_server *sv, int slots);
+
 /* Seconds since the server was created (the clock of t_submit, t_admit, t_done). */
 int e3x_server_clock(const e3x_server *sv, double *seconds);
 

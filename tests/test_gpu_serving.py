@@ -170,6 +170,30 @@ def test_admission_by_memory_and_arrivals_during_work():
         _check_same(got[i], _alone(q))
 
 
+def test_small_grids_have_slots_of_their_own():
+    """With small slots, requests on small grids (at most the co-batching size) take slots
+    beside max_active: a large solve holds the only regular slot and a small request
+    submitted after it starts at once; without small slots it waits.  Results equal the
+    solves alone."""
+    large = dict(_requests(1, seed=14)[0], shape=(24, 20, 16))
+    large["pix"], large["rod"] = voronoi_case(large["shape"], large["nphase"] - 1, seed=1401, pore_frac=0.02)
+    small = dict(_requests(2, seed=15)[1], shape=(8, 8, 8))
+    small["pix"], small["rod"] = voronoi_case(small["shape"], small["nphase"] - 1, seed=1501, pore_frac=0.02)
+    started = {}
+    for slots in (0, 1):
+        with Server(max_active=1, batch_members=6, batch_max_nodes=12 ** 3, small_slots=slots) as sv:
+            ids = [_submit(sv, large), _submit(sv, small)]
+            active, queued, _ = sv.progress()
+            started[slots] = (active, queued)
+            sv.drain()
+            got = {r["id"]: r for r in sv.take()}
+        _check_same(got[ids[0]], _alone(large))
+        _check_same(got[ids[1]], _alone(small))
+    assert started[0] == (1, 1) and started[1] == (2, 0)
+    with Server(max_active=1) as sv, pytest.raises(L.E3xError):
+        sv.set_small_slots(-1)
+
+
 def test_bad_request_is_reported_not_fatal():
     qs = _requests(2, seed=13)
     bad = dict(qs[0])
