@@ -46,9 +46,9 @@ hipError_t dalloc(Solver &s, T **p, int64_t n)
 void release(Solver &s)
 {
     (void)hipSetDevice(s.device);
-    void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.perm, s.u, s.g, s.b, s.h, s.h_alt, s.z,
-                   s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.du32, s.cconst, s.partials,
-                   s.st, s.hist, s.results, s.post, s.hot, s.hphase, s.minv_hom, s.minv_hom32};
+    void *dev[] = {s.cmod, s.dk, s.sten, s.dk32, s.sten32, s.orient, s.pix, s.perm, s.tcnt, s.u, s.g, s.b, s.h,
+                   s.h_alt, s.z, s.ah, s.minv, s.g32, s.h32, s.h32_alt, s.z32, s.ah32, s.minv32, s.du32, s.cconst,
+                   s.partials, s.st, s.hist, s.results, s.post, s.hot, s.hphase, s.minv_hom, s.minv_hom32};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (s.results_host) (void)hipHostFree(s.results_host);
@@ -342,6 +342,7 @@ int solver_create(int nx, int ny, int nz, int nphase, int nrhs, int members, int
     const int64_t nm = s->member_count();      // microstructures held
     CK(dalloc(*s, reinterpret_cast<uint8_t **>(&s->pix), id_bytes * ns * nm));
     CK(dalloc(*s, &s->perm, stencil_blocks(*s) * kSB * nm));
+    CK(dalloc(*s, &s->tcnt, stencil_blocks(*s) * nm));
     CK(dalloc(*s, &s->u, nv));
     CK(dalloc(*s, &s->g, nv));
     CK(dalloc(*s, &s->b, nv));

@@ -111,6 +111,9 @@ struct Solver {
     // the tile's wave-cube numbering): homogeneous nodes by phase, then mixed nodes by
     // their phases, so that waves hold few distinct phases; built on upload
     PermT *perm = nullptr;                                     // [tiles][kSB]
+    // per stencil tile, its homogeneous (low 16 bits) and mixed (high 16 bits) node counts:
+    // the lane order holds the mixed nodes at positions [homogeneous, homogeneous + mixed)
+    unsigned *tcnt = nullptr;                                  // [tiles]
 
     // fields, [nrhs][ns][3] (the three components of a node side by side)
     double *u = nullptr, *g = nullptr, *b = nullptr, *h = nullptr, *z = nullptr, *ah = nullptr;

@@ -184,7 +184,8 @@ int64_t footprint(int nx, int ny, int nz, int nphase_cap, bool mixed, int ldemb)
     const int64_t ns = (int64_t)nx * ny * nz;
     const int64_t id = nphase_cap > kNarrowPhases ? 4 : 2;  // phase-id width
     int64_t b = ns * id;                                  // pix
-    b += (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ) * kSB * (int64_t)sizeof(PermT);  // lane order
+    b += (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ) *
+         (kSB * (int64_t)sizeof(PermT) + (int64_t)sizeof(unsigned));  // lane order and node counts
     b += 7 * 3 * ns * 8 + 6 * ns * 8 + ns * id;           // u g b h h_alt z Ah, M^-1, homogeneous phase
     b += (int64_t)nphase_cap * ((36 + 576 + kStenHalf + 3 + 6) * 8 + (576 + kStenHalf + 6) * 4);
     // partials (the larger of the streaming and the stencil reductions), state

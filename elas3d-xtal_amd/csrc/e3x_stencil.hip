@@ -11,15 +11,17 @@
 //   - homogeneous lanes: the wave loops over the phases its lanes hold; for phase P the
 //     27-point stencil of P (243 FMA; the block of -delta is the transpose of the block
 //     of +delta, so 126 coefficients) goes to the lanes of phase P;
-//   - mixed lanes: for each octant, in the pairs (a, a^6) of opposite corners, the wave
-//     reads the 24 neighbour values of that voxel once, then loops over the phases its
+//   - mixed nodes: shared by the block's four waves, wave w taking the octant pair
+//     (w, w^6) of opposite corners for every mixed node of the tile: per octant a lane
+//     reads the 24 neighbour values of that voxel, the wave loops over the phases its
 //     lanes have there and applies 3x24 rows of the element matrix (72 FMA) to those
-//     lanes; both octants of a pair use the rows dk[P][a], the second in permuted order.
+//     lanes; both octants of a pair use the rows dk[P][w], the second in permuted order.
+//     The lane holding the node adds the four pair sums in wave order.
 // The next phase is the one of the first lane still pending (ballot + readlane).  Each
 // lane is served exactly once per octant (or once in all, when homogeneous), so a
 // node's sum runs over its octants in fixed order whatever its neighbours hold: results
 // are bitwise reproducible.  Each block serves one right-hand side, so the
-// kernel is the same for any number of them (22 KB of LDS in FP64).
+// kernel is the same for any number of them (24 KB of LDS in FP64).
 #include "e3x_device.hpp"
 
 #include <cmath>
@@ -54,6 +56,7 @@ struct StencilArgs {
     int64_t ns;
     const void *pix;       // phase ids: uint16, or uint32 on a wide handle (the kernel's PhT)
     const PermT *perm;     // lane order per tile (Solver::perm)
+    const unsigned *tcnt;  // homogeneous and mixed node counts per tile (Solver::tcnt)
     int64_t mstride;       // batch: nodes between the members' phase maps (0: one shared map)
     const T *dk, *sten;
     const T *x;            // [nrhs][ns][3]
@@ -160,6 +163,8 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
     __shared__ T hs[3 * HN];
     __shared__ T as[3 * kSB];  // Ah in x-fastest tile order
     __shared__ int ps[VN];
+    __shared__ PermT tps[kSB];   // the lane order (node of each lane)
+    __shared__ T part[4][3][64];  // mixed nodes: each wave's partial sums
     __shared__ unsigned arrived;  // waves done (last_wave)
 
     // grid (tiles, right-hand sides); the XCD remap runs over the linear block id, whose
@@ -259,6 +264,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
 #pragma unroll
     for (int k = 0; k < kVoxPer; k++)
         if (tid + k * kSB < VN) ps[tid + k * kSB] = pvl[k];
+    tps[tid] = (PermT)t;
     if (stop != 0) return;  // uniform over the block
     const int q = hpos_index(pin);
     int qs[kShellPer];
@@ -307,7 +313,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
     bool homog = true;
 #pragma unroll
     for (int a = 1; a < 8; a++) homog &= pv[a] == pv[0];
-    const bool hlane = active && homog, mlane = active && !homog;
+    const bool hlane = active && homog;
     const T *hr = hs + (lz + 1) * (HX * HY) + (ly + 1) * HX + (lx + 1);
 
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);
@@ -346,77 +352,133 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
             }
         }
     }
-    if (__ballot(mlane)) {
-        // Octants in pairs (a, a ^ 6), a = 0..3: corner a ^ 6 is the one opposite a, and the
-        // element matrix is symmetric under inversion through the voxel centre
-        // (dk[P][a^6][i][b][j] = dk[P][a][i][b^6][j], the elasticity tensor being even), so
-        // the second octant of a pair reads the first one's coefficient rows with its node
-        // blocks taken in b ^ 6 order.  Half the table is read, and each slice twice in a
-        // row: the second pass finds it in the scalar cache.
+    // Mixed nodes, shared by the four waves: the tile's M mixed nodes sit at positions
+    // [H, H + M) of the lane order.  Wave w applies the octant pair (w, w ^ 6) to each of
+    // them, 64 at a time (lane l takes the node at position H + j0 + l), and leaves its
+    // partial sum in LDS; after a barrier the lane holding the node adds the four partials
+    // in wave order.  A tile's mixed work then runs on all four SIMDs at once instead of
+    // keeping the block (and its LDS) resident while the one wave that holds the mixed
+    // nodes goes through all eight octants.  Corner a ^ 6 is the one opposite a, and the
+    // element matrix is symmetric under inversion through the voxel centre
+    // (dk[P][a^6][i][b][j] = dk[P][a][i][b^6][j], the elasticity tensor being even), so
+    // the second octant of a pair reads the first one's coefficient rows with its node
+    // blocks taken in b ^ 6 order: the pair reads one slice, the second pass finds it in
+    // the scalar cache.  For each octant the wave loops over the phases its lanes hold
+    // there (ballot of the pending lanes, phase of the first one), so every lane is served
+    // exactly once per octant and a node's sum runs in fixed order: results are bitwise
+    // reproducible whatever the tile's other nodes hold.
+    const unsigned hm = A.tcnt[(A.mstride != 0 ? (int64_t)r * ntile : 0) + bid];
+    const int hcount = (int)(hm & 0xFFFFu), mcount = (int)(hm >> 16);
+    if (mcount != 0) {  // uniform over the block
+        static_assert(kSB == 256, "one octant pair per wave");
+        // the wave's octant pair, uniform (so the coefficient loads stay scalar)
+        const int a = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
 #pragma unroll 1
-        for (int k = 0; k < 8; k++) {
-            const int a = k >> 1, oct = (k & 1) ? (a ^ 6) : a;
-            // voxel oct of the node spans nodes (PX(b)-PX(oct), ...), b = 0..7
-            const T *hb = hr - (PZ(oct) * (HX * HY) + PY(oct) * HX + PX(oct));
-            T hv[24];
-            if (k & 1) {
+        for (int j0 = 0; j0 < mcount; j0 += 64) {
+            const int mi = j0 + l;
+            const bool mine = mi < mcount;
+            const int tm = tps[hcount + min(mi, mcount - 1)];
+            const int mw = tm >> 6, ml = tm & 63;
+            const int mx = wave_x(mw) + (ml & 3), my = wave_y(mw) + ((ml >> 2) & 3), mz = wave_z(mw) + (ml >> 4);
+            const int *pcm = ps + (mz + 1) * (VX * VY) + (my + 1) * VX + (mx + 1);
+            const T *hrm = hs + (mz + 1) * (HX * HY) + (my + 1) * HX + (mx + 1);
+            T p0 = T(0), p1 = T(0), p2 = T(0);  // FP32: the pair's partial sum
+#pragma unroll 1
+            for (int half = 0; half < 2; half++) {
+                const int oct = half ? (a ^ 6) : a;
+                // voxel oct of the node spans nodes (PX(b)-PX(oct), ...), b = 0..7
+                const T *hb = hrm - (PZ(oct) * (HX * HY) + PY(oct) * HX + PX(oct));
+                T hv[24];
+                if (half) {
 #pragma unroll
-                for (int b = 0; b < 8; b++)
+                    for (int b = 0; b < 8; b++)
 #pragma unroll
-                    for (int n = 0; n < 3; n++)
-                        hv[b * 3 + n] = hb[n * HN + PZ(b ^ 6) * (HX * HY) + PY(b ^ 6) * HX + PX(b ^ 6)];
-            } else {
+                        for (int n = 0; n < 3; n++)
+                            hv[b * 3 + n] = hb[n * HN + PZ(b ^ 6) * (HX * HY) + PY(b ^ 6) * HX + PX(b ^ 6)];
+                } else {
 #pragma unroll
-                for (int b = 0; b < 8; b++)
+                    for (int b = 0; b < 8; b++)
 #pragma unroll
-                    for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
-            }
-            const int pa = pc[-(PZ(oct) * (VX * VY) + PY(oct) * VX + PX(oct))];
-            if constexpr (sizeof(T) == 8) {
-                // the slice of the octant's first phase (those of the others mostly follow
-                // from the octant before, or share its lines)
-                const int P0 = __builtin_amdgcn_readlane(pa, __builtin_ctzll(__ballot(mlane)));
-                touch_lines<72 * 8>(dk + (int64_t)P0 * 576 + a * 72);
-            }
-            bool pend = mlane;
-            while (true) {
-                const unsigned long long pm = __ballot(pend);
-                if (pm == 0ull) break;
-                const int P = __builtin_amdgcn_readlane(pa, __builtin_ctzll(pm));
-                ctab<T> *D = as_table(dk) + (int64_t)P * 576 + a * 72;
-                T t0 = T(0), t1 = T(0), t2 = T(0);
-#pragma unroll
-                for (int j = 0; j < 24; j++) {
-                    const T d0 = D[j], d1 = D[24 + j], d2 = D[48 + j];
-                    t0 += d0 * hv[j]; t1 += d1 * hv[j]; t2 += d2 * hv[j];
+                        for (int n = 0; n < 3; n++) hv[b * 3 + n] = hb[n * HN + PZ(b) * (HX * HY) + PY(b) * HX + PX(b)];
                 }
-                if (pend && pa == P) {
-                    acc0 += t0; acc1 += t1; acc2 += t2;
-                    pend = false;
+                const int pa = pcm[-(PZ(oct) * (VX * VY) + PY(oct) * VX + PX(oct))];
+                if constexpr (sizeof(T) == 8) {
+                    // the slice of the octant's first phase (those of the others mostly
+                    // follow from the octant before, or share its lines)
+                    const int P0 = __builtin_amdgcn_readlane(pa, __builtin_ctzll(__ballot(mine)));
+                    touch_lines<72 * 8>(dk + (int64_t)P0 * 576 + a * 72);
+                }
+                bool pend = mine;
+                while (true) {
+                    const unsigned long long pm = __ballot(pend);
+                    if (pm == 0ull) break;
+                    const int P = __builtin_amdgcn_readlane(pa, __builtin_ctzll(pm));
+                    ctab<T> *D = as_table(dk) + (int64_t)P * 576 + a * 72;
+                    T t0 = T(0), t1 = T(0), t2 = T(0);
+#pragma unroll
+                    for (int j = 0; j < 24; j++) {
+                        const T d0 = D[j], d1 = D[24 + j], d2 = D[48 + j];
+                        t0 += d0 * hv[j]; t1 += d1 * hv[j]; t2 += d2 * hv[j];
+                    }
+                    if (pend && pa == P) {
+                        if constexpr (sizeof(T) == 8) {
+                            // FP64: the pair's partial sum goes straight to LDS (six fewer
+                            // live registers keep the body at 80, six waves per SIMD)
+                            if (half) {
+                                part[a][0][l] += t0; part[a][1][l] += t1; part[a][2][l] += t2;
+                            } else {
+                                part[a][0][l] = t0; part[a][1][l] = t1; part[a][2][l] = t2;
+                            }
+                        } else {
+                            p0 += t0; p1 += t1; p2 += t2;
+                        }
+                        pend = false;
+                    }
                 }
             }
+            if constexpr (sizeof(T) != 8) {
+                part[a][0][l] = p0;
+                part[a][1][l] = p1;
+                part[a][2][l] = p2;
+            }
+            __syncthreads();
+            const int own = tid - hcount - j0;
+            if (own >= 0 && own < 64 && own + j0 < mcount) {
+                acc0 = part[0][0][own]; acc1 = part[0][1][own]; acc2 = part[0][2][own];
+#pragma unroll
+                for (int w = 1; w < 4; w++) {
+                    acc0 += part[w][0][own]; acc1 += part[w][1][own]; acc2 += part[w][2][own];
+                }
+            }
+            if (j0 + 64 < mcount) __syncthreads();  // before the next round overwrites part
         }
     }
 
     // --- store ---------------------------------------------------------------------
+    // The lane's node again from its lane-order entry (recomputed rather than kept live
+    // through the FMA passes, which need the registers)
+    int ts_ = t;
+    asm volatile("" : "+v"(ts_));
+    const int sw = ts_ >> 6, sl = ts_ & 63;
+    const int sx = wave_x(sw) + (sl & 3), sy = wave_y(sw) + ((sl >> 2) & 3), sz = wave_z(sw) + (sl >> 4);
+    const T *shr = hs + (sz + 1) * (HX * HY) + (sy + 1) * HX + (sx + 1);
     if constexpr (DIRECT) {
         // FP64 PCG apply: each lane stores its own node's Ah and leaves only the node's
         // h.Ah in LDS, at the node's x-fastest place; the last wave of the block sums those
         // as the tile epilogue below does (each lane over its four nodes in turn, then the
-        // wave), so the partial is the same bits.  The block holds 18 KB of LDS instead of
-        // 22, so more blocks stay resident while a block's slowest (mixed) wave finishes.
+        // wave), so the partial is the same bits.  The block needs no Ah tile in LDS.
         __shared__ double prs[kSB];
-        if (active) {
-            const int64_t m = o + 3 * (int64_t)((ox + lx) + nx * (oy + ly) + nxy * (oz + lz));
+        if (ox + sx < nx && oy + sy < ny && oz + sz < nz) {
+            const int64_t m = o + 3 * (int64_t)((ox + sx) + nx * (oy + sy) + nxy * (oz + sz));
             y[m] = acc0;
             y[m + 1] = acc1;
             y[m + 2] = acc2;
         }
         double pr = 0.0;
-        pr += (double)hr[0] * (double)acc0;
-        pr += (double)hr[HN] * (double)acc1;
-        pr += (double)hr[2 * HN] * (double)acc2;
-        prs[lx + TX * (ly + TY * lz)] = pr;
+        pr += (double)shr[0] * (double)acc0;
+        pr += (double)shr[HN] * (double)acc1;
+        pr += (double)shr[2 * HN] * (double)acc2;
+        prs[sx + TX * (sy + TY * sz)] = pr;
         if (!last_wave<kSB>(&arrived)) return;
         const int l = tid & 63;
         double p0 = 0.0;
@@ -435,7 +497,7 @@ __device__ __forceinline__ void stencil_body(const StencilArgs<T> &A, const PhT 
     // to get here stores the tile row by row (the lanes hold their nodes in phase order)
     // and forms the partial sums, each lane over its four nodes in turn, then the wave.
     {
-        const int nn = lx + TX * (ly + TY * lz);
+        const int nn = sx + TX * (sy + TY * sz);
         as[nn] = acc0; as[kSB + nn] = acc1; as[2 * kSB + nn] = acc2;
     }
     if (!last_wave<kSB>(&arrived)) return;
@@ -515,7 +577,7 @@ static_assert(VN <= kSortN && kSortN % kSB == 0, "voxel halo must fit the rank t
 
 template <typename PhT>
 __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int ntx, int nty, const PhT *pix,
-                                                  PermT *perm)
+                                                  PermT *perm, unsigned *tcnt)
 {
     __shared__ unsigned ps[VN];
     __shared__ unsigned srt[kSortN];
@@ -579,7 +641,9 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
     key[t] = (cls << 62) | ((unsigned long long)phase_rank(pmin) << 50) |
              ((unsigned long long)phase_rank(pmax) << 40) | ((unsigned long long)sig << 32) |
              ((unsigned long long)phase_rank(p0) << 22) | (unsigned long long)t;
-    __syncthreads();
+    // the tile's homogeneous and mixed node counts (barriers, as the key sort needs)
+    const unsigned nhom = (unsigned)__syncthreads_count(cls == 0ull), nmix = (unsigned)__syncthreads_count(cls == 1ull);
+    if (t == 0) tcnt[bid] = nhom | (nmix << 16);
     for (int k = 2; k <= kSB; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             const int i = t, p = t ^ j;
@@ -662,7 +726,7 @@ StencilArgs<T> stencil_args(const Solver &s, const T *dk, const T *sten, const T
     A.d_ntx = fast_div((unsigned)A.ntx);
     A.d_nty = fast_div((unsigned)A.nty);
     A.wrap1 = s.nx >= 8 && s.ny >= 8 && s.nz >= 8;
-    A.ns = s.ns; A.pix = s.pix; A.perm = s.perm; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
+    A.ns = s.ns; A.pix = s.pix; A.perm = s.perm; A.tcnt = s.tcnt; A.dk = dk; A.sten = sten; A.x = x; A.y = y;
     A.mstride = s.mstride();
     A.partials = s.partials; A.st = s.st;
     for (int r = 0; r < kMaxRhs; r++) A.rmap[r] = r;
@@ -806,12 +870,13 @@ hipError_t launch_tile_order(Solver &s, int member)
     const int ntx = (s.nx + TX - 1) / TX, nty = (s.ny + TY - 1) / TY;
     const int64_t mb = (int64_t)member * s.mstride();                    // member's phase map
     PermT *perm = s.perm + (s.members ? (int64_t)member * stencil_blocks(s) * kSB : 0);
+    unsigned *tcnt = s.tcnt + (s.members ? (int64_t)member * stencil_blocks(s) : 0);
     if (s.wide)
         hipLaunchKernelGGL(k_tile_order<uint32_t>, g, b, 0, s.stream, s.nx, s.ny, s.nz, ntx, nty,
-                           (const uint32_t *)s.pix + mb, perm);
+                           (const uint32_t *)s.pix + mb, perm, tcnt);
     else
         hipLaunchKernelGGL(k_tile_order<uint16_t>, g, b, 0, s.stream, s.nx, s.ny, s.nz, ntx, nty,
-                           (const uint16_t *)s.pix + mb, perm);
+                           (const uint16_t *)s.pix + mb, perm, tcnt);
     return hipGetLastError();
 }
 
