@@ -641,9 +641,25 @@ __global__ __launch_bounds__(kSB) void k_tile_order(int nx, int ny, int nz, int 
     key[t] = (cls << 62) | ((unsigned long long)phase_rank(pmin) << 50) |
              ((unsigned long long)phase_rank(pmax) << 40) | ((unsigned long long)sig << 32) |
              ((unsigned long long)phase_rank(p0) << 22) | (unsigned long long)t;
-    // the tile's homogeneous and mixed node counts (barriers, as the key sort needs)
-    const unsigned nhom = (unsigned)__syncthreads_count(cls == 0ull), nmix = (unsigned)__syncthreads_count(cls == 1ull);
-    if (t == 0) tcnt[bid] = nhom | (nmix << 16);
+    // the tile's homogeneous and mixed node counts: per wave by ballot, then summed by
+    // thread 0 after the barrier the key sort needs anyway
+    __shared__ unsigned wcnt[kSB / 64][2];
+    {
+        const unsigned long long bh = __ballot(cls == 0ull), bm = __ballot(cls == 1ull);
+        if ((t & 63) == 0) {
+            wcnt[t >> 6][0] = (unsigned)__popcll(bh);
+            wcnt[t >> 6][1] = (unsigned)__popcll(bm);
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        unsigned nhom = 0, nmix = 0;
+        for (int w = 0; w < kSB / 64; w++) {
+            nhom += wcnt[w][0];
+            nmix += wcnt[w][1];
+        }
+        tcnt[bid] = nhom | (nmix << 16);
+    }
     for (int k = 2; k <= kSB; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             const int i = t, p = t ^ j;
