@@ -448,6 +448,10 @@ int e3x_set_member(e3x_solver *h, int member, const int32_t *pix, const double *
         return fail(E3X_EINVAL, "orientation is NULL");
     }
     E3X_HIP(launch_tile_order(s, member));
+    // the ids staged in p16 / p32 (and the caller's orientations) must have reached the
+    // device before they go out of scope: an asynchronous copy from pageable memory may
+    // still be reading them when this call returns
+    E3X_HIP(hipStreamSynchronize(s.stream));
     double pm[4] = {0, 0, 0, 0};
     if (phasemod) {
         pm[0] = phasemod[0];
