@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -33,6 +34,16 @@ int fail(int code, const std::string &msg)
                         std::string(#call) + ": " + hipGetErrorString(e_));                     \
     } while (0)
 
+// E3X_POISON_ALLOC=1 (read at each allocation; a debugging aid, off by default): every
+// device allocation starts as 0xFF bytes (NaN in FP32 and FP64, the all-ones phase id),
+// so that a kernel that reads a buffer before anything wrote it gives results that differ
+// from a normal run's instead of depending on what the memory held before.
+bool poison_alloc()
+{
+    const char *v = std::getenv("E3X_POISON_ALLOC");
+    return v && v[0] == '1';
+}
+
 template <typename T>
 hipError_t dalloc(Solver &s, T **p, int64_t n)
 {
@@ -40,6 +51,10 @@ hipError_t dalloc(Solver &s, T **p, int64_t n)
     if (n <= 0) return hipSuccess;
     hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
     if (e == hipSuccess) s.bytes += (int64_t)sizeof(T) * n;
+    if (e == hipSuccess && poison_alloc()) {
+        e = hipMemset(*p, 0xFF, sizeof(T) * (size_t)n);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
     return e;
 }
 
@@ -366,11 +381,15 @@ int solver_create(int nx, int ny, int nz, int nphase, int nrhs, int members, int
     CK(hipHostMalloc((void **)&s->chk_host, sizeof(PcgState) * 2 * kMaxRhs, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&s->st_host, sizeof(PcgState) * 3 * kMaxRhs, hipHostMallocDefault));
     // fields read before anything writes them (h and z by a fused apply timed before any
-    // PCG, e3x_time_apply) start as zeros
-    CK(hipMemset(s->h, 0, sizeof(double) * (size_t)nv));
-    CK(hipMemset(s->h_alt, 0, sizeof(double) * (size_t)nv));
-    CK(hipMemset(s->z, 0, sizeof(double) * (size_t)nv));
-    CK(hipMemset(s->u, 0, sizeof(double) * (size_t)nv));
+    // PCG, e3x_time_apply) start as zeros.  The clears go on the handle's own stream and
+    // are waited for: that stream is non-blocking, so a plain hipMemset (null stream, and
+    // free to return before it completes) is ordered with none of the handle's later work
+    // and could land after k_affine has written the initial guess into u.
+    CK(hipMemsetAsync(s->h, 0, sizeof(double) * (size_t)nv, s->stream));
+    CK(hipMemsetAsync(s->h_alt, 0, sizeof(double) * (size_t)nv, s->stream));
+    CK(hipMemsetAsync(s->z, 0, sizeof(double) * (size_t)nv, s->stream));
+    CK(hipMemsetAsync(s->u, 0, sizeof(double) * (size_t)nv, s->stream));
+    CK(hipStreamSynchronize(s->stream));
 #undef CK
     *out = s;
     return E3X_OK;

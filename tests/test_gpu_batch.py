@@ -100,3 +100,41 @@ def test_batch_calls_and_labels():
     with pytest.raises(L.E3xError) as e:
         Elas3D(shape, batch_nphase=[3])            # a batch has 2..6 members
     assert e.value.code == L.E3X_EINVAL
+
+
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_poisoned_allocations_give_the_same_bits(monkeypatch, precision):
+    """Every device allocation filled with 0xFF bytes (NaN in FP32 and FP64, the all-ones
+    phase id; E3X_POISON_ALLOC=1) changes no bit of a batch or of a single solve: no kernel
+    reads a buffer before something wrote it."""
+    shape = (16, 12, 8)
+    mem = _members(shape, 4, seed=95)
+    m = mem[0]
+
+    def run():
+        batch, s = solve_batch(shape, mem, precision=precision, replace_every=7, keep=True)
+        try:
+            ub = [s.get(L.FIELD_U, r) for r in range(len(mem))]
+        finally:
+            s.close()
+        one, s1 = solve(m["pix"], m["orientation"], shape, m["nphase"], m["c11"], m["c12"], m["c44"], m["strain6"],
+                        precision=precision, replace_every=7, keep=True)
+        try:
+            u1 = s1.get(L.FIELD_U)
+        finally:
+            s1.close()
+        return batch, ub, one, u1
+
+    clean = run()
+    monkeypatch.setenv("E3X_POISON_ALLOC", "1")
+    poisoned = run()
+    monkeypatch.delenv("E3X_POISON_ALLOC")
+    for res in (clean, poisoned):
+        assert all(int(e) == L.PCG_CONVERGED for e in res[0].exit) and int(res[2].exit[0]) == L.PCG_CONVERGED
+    (b0, u0, o0, v0), (b1, u1, o1, v1) = clean, poisoned
+    assert np.array_equal(b0.lstep, b1.lstep) and np.array_equal(b0.utot, b1.utot)
+    assert np.array_equal(b0.stress6, b1.stress6)
+    assert all(np.array_equal(a, b) for a, b in zip(b0.history, b1.history))  # one array per member
+    assert all(np.array_equal(a, b) for a, b in zip(u0, u1))
+    assert o0.lstep[0] == o1.lstep[0] and o0.utot[0] == o1.utot[0] and np.array_equal(v0, v1)
+    assert np.array_equal(o0.stress6, o1.stress6)
